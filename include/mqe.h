@@ -1,0 +1,172 @@
+/*
+ * mqe.h -- C ABI of the MI355X walker engine for MPS projector QMC.
+ *
+ * The engine owns one GPU-resident walker population and replaces the
+ * per-walker CPU propagation of the reference's MPSQMC2 driver.  Every call
+ * returns 0 on success or a negative MQE_E* code, never throws, and leaves a
+ * message retrievable with mqe_last_error().  Host arrays are borrowed for the
+ * duration of the call only.  One host thread per engine; calls on one engine
+ * are ordered on its stream.
+ *
+ * Layout conventions (identical to the reference):
+ *   MPS site tensor k:  A[row + dimL*(col + dimR*s)], dimL = vd[k], dimR = vd[k+1]
+ *                       (MPStensor.cpp:91-95, :168-173); a flat MPS is the site
+ *                       tensors concatenated in site order.
+ *   couplings:          listed lexicographically over (first < second)
+ *                       (MPSQMC2.cpp:286-299).
+ *   Trotter tables:     per coupling c, S[c*4 + k], U[c*16 + i + d*(j + d*k)],
+ *                       VT[c*16 + k + d*d*(i + d*j)]  (TrotterHeisenberg.h:83-90).
+ *   grid:               grid[i + dim*g]  (GridGenerator.h:48-49).
+ */
+#ifndef MQE_H
+#define MQE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MQE_OK 0
+#define MQE_EINVAL (-1)     /* bad argument / shape */
+#define MQE_EDEVICE (-2)    /* HIP runtime error */
+#define MQE_ECAPACITY (-3)  /* population exceeds the walker capacity */
+#define MQE_ENUMERIC (-4)   /* zero PDF norm or non-finite weight on some walker */
+#define MQE_ESTATE (-5)     /* call out of order (e.g. propagate before load) */
+
+#define MQE_PREC_FP64 0     /* all contractions FP64 (parity mode) */
+#define MQE_PREC_FP32 1     /* contractions FP32, QR/overlap/weights/energy FP64 */
+
+typedef struct mqe_engine mqe_engine;
+
+typedef struct {
+    int L;                 /* chain length */
+    int d;                 /* local dimension; the engine supports d = 2 */
+    int precision;         /* MQE_PREC_FP64 or MQE_PREC_FP32 */
+    int n_desired;         /* totalNDesiredWalkers (MPSQMC2.h:54) */
+    int capacity;          /* walker slots; the reference uses max(1000, 3N) (MPSQMC2.cpp:84) */
+    int max_bond;          /* largest walker bond dimension any walker will hold */
+    double dtau;           /* imaginary time step */
+    double field;          /* h of -h sum_i S^z_i (HeisenbergMPO.h:98-103) */
+    unsigned long long seed; /* device RNG key (used when no host uniforms are given) */
+} mqe_params;
+
+typedef struct {
+    int L;
+    int d;
+    const int *vd;         /* L+1 bond dimensions, vd[0] = vd[L] = 1 */
+    const double *data;    /* concatenated site tensors */
+} mqe_mps;
+
+/* Replaces MPSQMC2::MPSQMC2(...) set-up of the trial and the propagator tables
+ * (MPSQMC2.cpp:68-103, :219-332).  The trial is used as given (it must already
+ * be left-normalized as SetupTrial does, MPSQMC2.cpp:239).  J[c] is the coupling
+ * of pair (first[c], second[c]). */
+int mqe_create(const mqe_params *p, const mqe_mps *trial, int n_couplings, const int *first,
+               const int *second, const double *J, const double *tab_S, const double *tab_U,
+               const double *tab_VT, const double *grid, int n_grid, int grid_dim, mqe_engine **out);
+
+void mqe_destroy(mqe_engine *e);
+
+/* Replaces MPSQMC2::SetupWalkers (MPSQMC2.cpp:381-441): n walkers, walker i has
+ * bond dims vd[i*(L+1) ...] and its tensors at data[offset_i ...] (walkers
+ * concatenated), weight w[i], overlap O[i], energy history eh[i]. */
+int mqe_load_walkers(mqe_engine *e, int n, const int *vd, const double *data, const double *w,
+                     const double *O, const double *eh);
+
+/* Replaces the EnergyFunctionAndHistory(1, &E, false) call at the start of
+ * MPSQMC2::Walk (MPSQMC2.cpp:447): local energies with the stored overlaps, no
+ * history update.  Also brings the walkers to the engine's canonical form. */
+int mqe_prepare(mqe_engine *e, double *E_start);
+
+/* Replaces MPSQMC2::PropagateSeparately (MPSQMC2.cpp:518-611) and computes the
+ * per-walker local energies of the propagated walkers (Walker.cpp:72-77).
+ * uniforms: n_walkers*n_couplings draws, walker-major, or NULL for the device
+ * RNG.  *sum_w receives the sum of the walker weights. */
+int mqe_propagate(mqe_engine *e, const double *uniforms, double *sum_w);
+
+/* Replaces MPSQMC2::EnergyFunctionAndHistory (MPSQMC2.cpp:688-747): E_P and
+ * Omega from the local energies of the last mqe_propagate. */
+int mqe_energy(mqe_engine *e, int step, int do_fluct, double *E_P, double *omega);
+
+/* Replaces MPSQMC2::SeparatePopulationControl (MPSQMC2.cpp:613-686).  The
+ * k-th walker (in population order) whose scaled weight lies outside
+ * [0.25, 1.5] consumes uniforms[k]; uniforms may be NULL for the device RNG.
+ * *n_new receives the new population size, *n_used the draws consumed. */
+int mqe_population_control(mqe_engine *e, double scaling, const double *uniforms, int n_uniforms,
+                           int *n_new, int *n_used);
+
+/* One full Walk step (MPSQMC2.cpp:465-511) on the device with the device RNG:
+ * propagate, scaling = n_desired / sum_w, E_T, energy with history, population
+ * control.  No host synchronisation; the per-step record (step, N, E_P, E_T,
+ * Omega, sum_w) is kept on the device and read with mqe_read_trace. */
+int mqe_step(mqe_engine *e, int step);
+
+/* Run steps first_step .. first_step+n_steps-1 with mqe_step; when
+ * use_graph != 0 the step is captured once in a hipGraph and replayed. */
+int mqe_run(mqe_engine *e, int first_step, int n_steps, int use_graph);
+
+/* Copy the trace rows of steps first_step .. first_step+n_steps-1 (6 doubles each). */
+int mqe_read_trace(mqe_engine *e, int first_step, int n_steps, double *out);
+
+int mqe_sync(mqe_engine *e);
+int mqe_num_walkers(mqe_engine *e, int *n);
+int mqe_num_couplings(mqe_engine *e, int *n);
+
+/* Walker scalars in population order (any pointer may be NULL). */
+int mqe_read_scalars(mqe_engine *e, double *w, double *O, double *eh, double *EL);
+
+/* Sampled grid point of every (walker, coupling) of the last mqe_propagate, walker-major. */
+int mqe_read_indices(mqe_engine *e, int *idx);
+
+/* Walker i (population order): bond dims and tensors (gauge is the engine's,
+ * the state is the same as the reference's up to gauge). size = flat length. */
+int mqe_read_walker(mqe_engine *e, int i, int *vd, double *data, long *size);
+
+/* out[i] = <other | walker_i> by transfer matrices (MPSstate.cpp:248-284),
+ * batched over the population; other may have any bond dimensions. */
+int mqe_overlaps(mqe_engine *e, const mqe_mps *other, double *out);
+
+/* Device time of the last propagate kernel in milliseconds (HIP events on the
+ * engine's stream), and the number of walkers it processed. */
+int mqe_last_kernel_ms(mqe_engine *e, double *ms, int *walkers);
+
+const char *mqe_last_error(void);
+
+/* ---- set-up helpers (host code; no GPU needed) ----------------------------- */
+
+/* Bond dimensions of MPSstate(L, D, d) (MPSstate.cpp:20-28). */
+void mqe_bond_dims(int L, int D, int d, int *vd);
+
+/* Operator-Schmidt tables of exp(-dtau J S_i.S_j) in the reference's storage
+ * (TrotterHeisenberg.cpp:102-285).  d = 2 uses the closed form in the
+ * Pauli basis {I, Sx, iSy, Sz}; returns MQE_EINVAL for other d. */
+int mqe_trotter_tables(int d, double J, double dtau, double *S, double *U, double *VT);
+
+/* Marsaglia grid on the unit sphere in dim dimensions with p points per
+ * coordinate (GridGenerator.cpp:167-225).  grid must hold dim*p^dim doubles;
+ * *n receives the number of points. */
+int mqe_marsaglia_grid(int dim, int p, double *grid, int *n);
+
+/* MPSstate::LeftNormalize (MPSstate.cpp:216-229) on a flat MPS, in place;
+ * *norm receives the final 1x1 R factor. */
+int mqe_left_normalize(int L, int d, const int *vd, double *data, double *norm);
+
+/* MPSstate::CompressState(Dt) (MPSstate.cpp:379-404): two SVD sweeps with the
+ * 1e-13 absolute Schmidt cutoff.  vd_out receives the new bond dims and out
+ * the tensors (capacity: flat size of the input). */
+int mqe_compress(int L, int d, const int *vd, const double *data, int Dt, int *vd_out, double *out);
+
+/* ---- host random stream (the build's RNG contract) ------------------------- */
+/* Random::rand (Random.cpp:49-59) with a fixed seed: one std::mt19937 stream,
+ * u = x * 2^-32.  peek fills out without consuming; discard consumes k draws. */
+typedef struct mqe_rng mqe_rng;
+int mqe_rng_create(unsigned int seed, mqe_rng **out);
+void mqe_rng_destroy(mqe_rng *r);
+int mqe_rng_uniforms(mqe_rng *r, long n, double *out);
+int mqe_rng_peek(mqe_rng *r, long n, double *out);
+int mqe_rng_discard(mqe_rng *r, long k);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MQE_H */

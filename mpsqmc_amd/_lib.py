@@ -1,0 +1,106 @@
+"""ctypes binding of the walker engine's C ABI (``include/mqe.h``)."""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmqe.so")
+
+c_int = ctypes.c_int
+c_double = ctypes.c_double
+c_long = ctypes.c_long
+P = ctypes.POINTER
+
+MQE_PREC_FP64 = 0
+MQE_PREC_FP32 = 1
+
+ERRORS = {
+    -1: "MQE_EINVAL",
+    -2: "MQE_EDEVICE",
+    -3: "MQE_ECAPACITY",
+    -4: "MQE_ENUMERIC",
+    -5: "MQE_ESTATE",
+}
+
+
+class MqeParams(ctypes.Structure):
+    _fields_ = [
+        ("L", c_int),
+        ("d", c_int),
+        ("precision", c_int),
+        ("n_desired", c_int),
+        ("capacity", c_int),
+        ("max_bond", c_int),
+        ("dtau", c_double),
+        ("field", c_double),
+        ("seed", ctypes.c_ulonglong),
+    ]
+
+
+class MqeMps(ctypes.Structure):
+    _fields_ = [("L", c_int), ("d", c_int), ("vd", P(c_int)), ("data", P(c_double))]
+
+
+class MqeError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+# symbol -> (restype, argtypes)
+SIGNATURES = {
+    "mqe_create": (c_int, [P(MqeParams), P(MqeMps), c_int, P(c_int), P(c_int), P(c_double), P(c_double),
+                           P(c_double), P(c_double), P(c_double), c_int, c_int, P(ctypes.c_void_p)]),
+    "mqe_destroy": (None, [ctypes.c_void_p]),
+    "mqe_load_walkers": (c_int, [ctypes.c_void_p, c_int, P(c_int), P(c_double), P(c_double), P(c_double),
+                                 P(c_double)]),
+    "mqe_prepare": (c_int, [ctypes.c_void_p, P(c_double)]),
+    "mqe_propagate": (c_int, [ctypes.c_void_p, P(c_double), P(c_double)]),
+    "mqe_energy": (c_int, [ctypes.c_void_p, c_int, c_int, P(c_double), P(c_double)]),
+    "mqe_population_control": (c_int, [ctypes.c_void_p, c_double, P(c_double), c_int, P(c_int), P(c_int)]),
+    "mqe_step": (c_int, [ctypes.c_void_p, c_int]),
+    "mqe_run": (c_int, [ctypes.c_void_p, c_int, c_int, c_int]),
+    "mqe_read_trace": (c_int, [ctypes.c_void_p, c_int, c_int, P(c_double)]),
+    "mqe_sync": (c_int, [ctypes.c_void_p]),
+    "mqe_num_walkers": (c_int, [ctypes.c_void_p, P(c_int)]),
+    "mqe_num_couplings": (c_int, [ctypes.c_void_p, P(c_int)]),
+    "mqe_read_scalars": (c_int, [ctypes.c_void_p, P(c_double), P(c_double), P(c_double), P(c_double)]),
+    "mqe_read_indices": (c_int, [ctypes.c_void_p, P(c_int)]),
+    "mqe_read_walker": (c_int, [ctypes.c_void_p, c_int, P(c_int), P(c_double), P(c_long)]),
+    "mqe_overlaps": (c_int, [ctypes.c_void_p, P(MqeMps), P(c_double)]),
+    "mqe_last_kernel_ms": (c_int, [ctypes.c_void_p, P(c_double), P(c_int)]),
+    "mqe_last_error": (ctypes.c_char_p, []),
+    "mqe_bond_dims": (None, [c_int, c_int, c_int, P(c_int)]),
+    "mqe_trotter_tables": (c_int, [c_int, c_double, c_double, P(c_double), P(c_double), P(c_double)]),
+    "mqe_marsaglia_grid": (c_int, [c_int, c_int, P(c_double), P(c_int)]),
+    "mqe_left_normalize": (c_int, [c_int, c_int, P(c_int), P(c_double), P(c_double)]),
+    "mqe_compress": (c_int, [c_int, c_int, P(c_int), P(c_double), c_int, P(c_int), P(c_double)]),
+    "mqe_rng_create": (c_int, [ctypes.c_uint, P(ctypes.c_void_p)]),
+    "mqe_rng_destroy": (None, [ctypes.c_void_p]),
+    "mqe_rng_uniforms": (c_int, [ctypes.c_void_p, c_long, P(c_double)]),
+    "mqe_rng_peek": (c_int, [ctypes.c_void_p, c_long, P(c_double)]),
+    "mqe_rng_discard": (c_int, [ctypes.c_void_p, c_long]),
+}
+
+
+def lib():
+    """Load libmqe.so (built in-tree).  Raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} not built; run python -m mpsqmc_amd.build or __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(code):
+    if code != 0:
+        msg = lib().mqe_last_error()
+        raise MqeError(code, msg.decode() if msg else "")
+    return code

@@ -1,0 +1,210 @@
+// mqe_device.h -- workgroup-cooperative device primitives of the walker engine.
+//
+// One 256-thread workgroup (four wave64s) owns one walker.  All matrices are
+// column-major with explicit leading dimensions and are addressed through
+// generic pointers, so the same routines run on global (L2-resident) scratch
+// and on LDS.
+#ifndef MQE_DEVICE_H
+#define MQE_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define MQE_NT 256
+#define MQE_WAVES (MQE_NT / 64)
+
+namespace mqe {
+
+// ---------------------------------------------------------------------------
+// reductions
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Sum over the workgroup; every thread receives the total.  sh: >= MQE_WAVES doubles.
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < MQE_WAVES; ++w) t += sh[w];
+    return t;
+}
+
+// K simultaneous sums; sh: >= K*MQE_WAVES doubles.
+template <int K>
+__device__ __forceinline__ void block_sum_n(double (&v)[K], double *sh) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) sh[wid * K + k] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < MQE_WAVES; ++w) t += sh[w * K + k];
+        v[k] = t;
+    }
+}
+
+// Inclusive prefix sum of x[0..n) into y[0..n) (double), whole workgroup.
+// sh: >= MQE_NT + MQE_WAVES doubles.
+__device__ __forceinline__ void block_inclusive_scan(int n, const double *x, double *y, double *sh) {
+    const int tid = threadIdx.x;
+    const int per = (n + MQE_NT - 1) / MQE_NT;
+    const int b0 = tid * per, b1 = min(n, b0 + per);
+    double loc = 0.0;
+    for (int i = b0; i < b1; ++i) loc += x[i];
+    // wave inclusive scan of the per-thread totals
+    const int lane = tid & 63, wid = tid >> 6;
+    double s = loc;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        double o = __shfl_up(s, off, 64);
+        if (lane >= off) s += o;
+    }
+    __syncthreads();
+    if (lane == 63) sh[MQE_NT + wid] = s;
+    __syncthreads();
+    double base = 0.0;
+    for (int w = 0; w < wid; ++w) base += sh[MQE_NT + w];
+    double run = base + s - loc;  // exclusive prefix of this thread's chunk
+    for (int i = b0; i < b1; ++i) { run += x[i]; y[i] = run; }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// GEMM: C(m x n) (+)= op(A) op(B).  op(A) is m x k: A(r,q) = TA ? A[q + lda*r] : A[r + lda*q].
+// ---------------------------------------------------------------------------
+template <bool TA, bool TB, typename T>
+__device__ __forceinline__ void gemm(int m, int n, int k, const T *A, int lda, const T *B, int ldb, T *C, int ldc,
+                                     bool acc) {
+    const int mn = m * n;
+    for (int e = threadIdx.x; e < mn; e += MQE_NT) {
+        const int r = e % m, c = e / m;
+        T s = acc ? C[r + ldc * c] : T(0);
+        const T *ap = TA ? (A + (size_t)lda * r) : (A + r);
+        const T *bp = TB ? (B + c) : (B + (size_t)ldb * c);
+        for (int q = 0; q < k; ++q) {
+            const T a = TA ? ap[q] : ap[(size_t)lda * q];
+            const T b = TB ? bp[(size_t)ldb * q] : bp[q];
+            s += a * b;
+        }
+        C[r + ldc * c] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Householder QR (LAPACK geqrf/orgqr conventions), FP64, whole workgroup.
+// A: m x n (m >= n), ld m.  On exit A holds Q and R (n x n, ld n) the upper
+// triangle.  tau: n; parts: MQE_NT; dots: n; sh: MQE_WAVES.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void apply_reflector(int m, int n, int j, double tj, double *A, double *parts,
+                                                double *dots) {
+    const int tid = threadIdx.x;
+    const int c0 = j + 1, nc = n - c0;
+    if (nc <= 0) return;
+    const int G = max(1, MQE_NT / nc);
+    if (tid < G * nc) {
+        const int cc = tid % nc, g = tid / nc;
+        const double *vj = A + (size_t)m * j;
+        const double *ac = A + (size_t)m * (c0 + cc);
+        double s = 0.0;
+        for (int i = j + 1 + g; i < m; i += G) s += vj[i] * ac[i];
+        parts[g * nc + cc] = s;
+    }
+    __syncthreads();
+    if (tid < nc) {
+        double s = A[j + (size_t)m * (c0 + tid)];
+        for (int g = 0; g < G; ++g) s += parts[g * nc + tid];
+        dots[tid] = tj * s;
+    }
+    __syncthreads();
+    const int rows = m - j;
+    for (int e = tid; e < nc * rows; e += MQE_NT) {
+        const int cc = e % nc, i = j + e / nc;
+        const double f = dots[cc];
+        A[i + (size_t)m * (c0 + cc)] -= f * (i == j ? 1.0 : A[i + (size_t)m * j]);
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void householder_qr(int m, int n, double *A, double *R, double *tau, double *parts,
+                                               double *dots, double *sh) {
+    const int tid = threadIdx.x;
+    for (int j = 0; j < n; ++j) {
+        double ss = 0.0;
+        for (int i = j + 1 + tid; i < m; i += MQE_NT) {
+            const double a = A[i + (size_t)m * j];
+            ss += a * a;
+        }
+        ss = block_sum(ss, sh);
+        const double alpha = A[j + (size_t)m * j];
+        double tj = 0.0, beta = alpha, scale = 0.0;
+        if (ss != 0.0) {
+            beta = -copysign(sqrt(alpha * alpha + ss), alpha);
+            tj = (beta - alpha) / beta;
+            scale = 1.0 / (alpha - beta);
+        }
+        __syncthreads();
+        if (tj != 0.0) {
+            for (int i = j + 1 + tid; i < m; i += MQE_NT) A[i + (size_t)m * j] *= scale;
+            if (tid == 0) A[j + (size_t)m * j] = beta;
+        }
+        if (tid == 0) tau[j] = tj;
+        __syncthreads();
+        if (tj != 0.0) apply_reflector(m, n, j, tj, A, parts, dots);
+    }
+    for (int e = tid; e < n * n; e += MQE_NT) {
+        const int r = e % n, c = e / n;
+        R[e] = (r <= c) ? A[r + (size_t)m * c] : 0.0;
+    }
+    __syncthreads();
+    for (int j = n - 1; j >= 0; --j) {
+        const double tj = tau[j];
+        if (tj != 0.0) apply_reflector(m, n, j, tj, A, parts, dots);
+        for (int i = tid; i < m; i += MQE_NT) {
+            const double v = A[i + (size_t)m * j];
+            A[i + (size_t)m * j] = (i < j) ? 0.0 : (i == j ? 1.0 - tj : -tj * v);
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) counter-based RNG.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t philox_u32(uint64_t key, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3) {
+    uint32_t k0 = (uint32_t)key, k1 = (uint32_t)(key >> 32);
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return c0;
+}
+
+__device__ __forceinline__ double philox_uniform(uint64_t key, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3) {
+    return (double)philox_u32(key, c0, c1, c2, c3) * (1.0 / 4294967296.0);
+}
+
+}  // namespace mqe
+
+#endif

@@ -1,0 +1,1619 @@
+// mqe_engine.hip -- MI355X walker engine for MPS projector QMC (gfx950).
+//
+// Replaces the per-walker CPU propagation of the reference's MPSQMC2 driver
+// (MPSQMC2.cpp:518-747).  All walkers live on one GPU.  One 256-thread
+// workgroup owns one walker for a whole step:
+//
+//   propagation sweep (left -> right), per coupling (i, j):
+//     G   = T_i^T L_i phi_i                       (4 trial/walker physical blocks)
+//     G  <- carried through sites i+1..j-1        (long-range couplings only)
+//     V   = T_j R_{j+1} phi_j^T                   (4 blocks)
+//     M   = <G, V>                                (16 numbers = all two-site
+//                                                   mixed matrix elements)
+//     combos, PDF over the grid, CDF sample, weight update (MPSQMC2.cpp:540-588)
+//     apply the sampled operator on sites i and j (MPSstate.cpp:575-633)
+//     normalise: left norm environment N (nearest-neighbour fast path) or an
+//     FP64 Householder LQ sweep over j..i+1 (long-range), overlap from M.
+//   energy sweep (right -> left):
+//     FP64 Householder LQ per site (right-canonical form for the next step,
+//     right environments R_k), bond energies <trial|S_i.S_j|phi> from the
+//     same two-site machinery, overlap and norm at site 0 (Walker.cpp:54-77).
+//
+// This environment algorithm yields the same overlaps, PDFs, weights and local
+// energies as the reference's 16+1 full transfer-matrix overlaps per coupling
+// (the quantities are gauge invariant), at O(L D^3) instead of O(L^2 D^3) per
+// walker-step.  T = double (parity mode) or float (FP32 contractions; QR,
+// sampling, weights, overlaps and energies stay FP64).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mqe.h"
+#include "mqe_device.h"
+#include "mqe_internal.h"
+
+#define MQE_MAXL 256
+#define MQE_MAXG 4096
+
+namespace mqe {
+
+static thread_local std::string g_err;
+void set_error(const char *msg) { g_err = msg ? msg : ""; }
+
+enum : int { F_PREP = 1, F_PROP = 2 };
+
+// Read-only description of the system (device pointers).
+struct Sys {
+    int L, nC, Ng, gdim, Dt, Dw;
+    const int *tvd;    // L+1 trial bond dims
+    const long *toff;  // L site offsets into the trial
+    const int *first, *second, *nextfirst;
+    const double *Jc;                // nC
+    const double *tS, *tU, *tVT;     // per coupling: 4, 16, 16
+    const double *grid;              // Ng * gdim
+    const int *cs_ptr, *cs_idx;      // couplings grouped by their second site (CSR, L+1)
+    double SS[16];                   // S.S two-site tensor, index (s'+2t')*4 + (s+2t)
+    double Sz[2];
+    double h;
+    double ssp[2];                   // exp(h dtau m / 2)
+};
+
+// The walker population (device pointers).
+struct Pool {
+    void *phi, *Renv, *Lenv;
+    int *wvd;
+    double *w, *O, *eh, *EL;
+    const int *order;
+    const int *n;
+    int *idx;
+    const double *u;       // host-supplied uniforms (n*nC, walker-major) or nullptr
+    long wflat, eflat;
+    const long *woff, *eoff;
+    unsigned long long seed;
+    int step;
+    int *status;
+    char *scratch;
+    long scratch_bytes;
+};
+
+// Scratch layout of one workgroup (sizes in elements).
+struct ScratchLayout {
+    long Y, G, GcA, GcB, W, V, N0, N1, TMP, LM, t_total;  // T region
+    long AQ, RQ, TAU, PARTS, DOTS, PDF, CUM, d_total;     // double region
+};
+
+__host__ __device__ inline ScratchLayout scratch_layout(int Dt, int Dw, int Ng) {
+    ScratchLayout s;
+    const long Dm = Dt > Dw ? Dt : Dw;
+    long o = 0;
+    s.Y = o; o += Dm * 2 * Dm;
+    s.G = o; o += 4L * Dt * Dw;
+    s.GcA = o; o += 4L * Dt * Dw;
+    s.GcB = o; o += 4L * Dt * Dw;
+    s.W = o; o += 2L * Dt * Dw;
+    s.V = o; o += 4L * Dt * Dw;
+    s.N0 = o; o += (long)Dw * Dw;
+    s.N1 = o; o += (long)Dw * Dw;
+    s.TMP = o; o += 2L * Dw * Dw;
+    s.LM = o; o += (long)Dw * Dw;
+    s.t_total = o;
+    o = 0;
+    s.AQ = o; o += 2L * Dw * Dw;
+    s.RQ = o; o += (long)Dw * Dw;
+    s.TAU = o; o += Dw;
+    s.PARTS = o; o += MQE_NT;
+    s.DOTS = o; o += (Dw > MQE_NT ? Dw : MQE_NT);
+    s.PDF = o; o += Ng;
+    s.CUM = o; o += Ng;
+    s.d_total = o;
+    return s;
+}
+
+template <typename T>
+__host__ __device__ inline long scratch_bytes_for(int Dt, int Dw, int Ng) {
+    ScratchLayout s = scratch_layout(Dt, Dw, Ng);
+    long tb = s.t_total * (long)sizeof(T);
+    tb = (tb + 255) & ~255L;
+    return tb + s.d_total * (long)sizeof(double) + 256;
+}
+
+// ---------------------------------------------------------------------------
+// per-walker context
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Ctx {
+    const Sys *S;
+    const T *trial;
+    T *phi, *Lenv, *Renv;
+    const long *woff, *eoff;
+    const int *tv, *wv;  // LDS copies of bond dims
+    ScratchLayout sl;
+    T *ts;               // T scratch
+    double *ds;          // double scratch
+    double *red;         // LDS reduction scratch (>= MQE_NT + 16*MQE_WAVES)
+
+    __device__ T *site(int k) const { return phi + woff[k]; }
+    __device__ const T *tsite(int k) const { return trial + S->toff[k]; }
+    __device__ T *Lk(int k) const { return Lenv + eoff[k]; }
+    __device__ T *Rk(int k) const { return Renv + eoff[k]; }
+};
+
+// Y = L_i phi_i (t_i x 2 w_{i+1});  G = T_i^T Y (2 t_{i+1} x 2 w_{i+1}), block
+// (s', s) = T_i[s']^T L_i phi_i[s] at G + s' t + s w (2t), ld 2t.
+template <typename T>
+__device__ void compute_G(const Ctx<T> &C, int i, const T *Lenv_i) {
+    const int ti = C.tv[i], tn = C.tv[i + 1], wi = C.wv[i], wn = C.wv[i + 1];
+    T *Y = C.ts + C.sl.Y, *G = C.ts + C.sl.G;
+    gemm<false, false, T>(ti, 2 * wn, wi, Lenv_i, ti, C.site(i), wi, Y, ti, false);
+    __syncthreads();
+    gemm<true, false, T>(2 * tn, 2 * wn, ti, C.tsite(i), ti, Y, ti, G, 2 * tn, false);
+    __syncthreads();
+}
+
+// W[t] = R_{j+1} phi_j[t]^T (t_{j+1} x w_j);  V[t'] = T_j[t'] W (t_j x 2 w_j);
+// block (t', t) at V + t' (t_j 2 w_j) + t (t_j w_j), ld t_j.
+template <typename T>
+__device__ void compute_V(const Ctx<T> &C, int j) {
+    const int tj = C.tv[j], tn = C.tv[j + 1], wj = C.wv[j], wn = C.wv[j + 1];
+    T *W = C.ts + C.sl.W, *V = C.ts + C.sl.V;
+    const T *R = C.Rk(j + 1), *ph = C.site(j);
+    for (int t = 0; t < 2; ++t)
+        gemm<false, true, T>(tn, wj, wn, R, tn, ph + (size_t)t * wj * wn, wj, W + (size_t)t * tn * wj, tn, false);
+    __syncthreads();
+    const T *Tj = C.tsite(j);
+    for (int tp = 0; tp < 2; ++tp)
+        gemm<false, false, T>(tj, 2 * wj, tn, Tj + (size_t)tp * tj * tn, tj, W, tn, V + (size_t)tp * tj * 2 * wj, tj,
+                              false);
+    __syncthreads();
+}
+
+// Carry four channels (t_k x w_k, ld t_k, contiguous at src) through site k
+// into dst (t_{k+1} x w_{k+1}).  Uses Y as temporary.
+template <typename T>
+__device__ void carry_channels(const Ctx<T> &C, int k, const T *src, T *dst) {
+    const int tk = C.tv[k], tn = C.tv[k + 1], wk = C.wv[k], wn = C.wv[k + 1];
+    T *Y = C.ts + C.sl.Y;
+    const T *Tk = C.tsite(k);
+    for (int ch = 0; ch < 4; ++ch) {
+        gemm<false, false, T>(tk, 2 * wn, wk, src + (size_t)ch * tk * wk, tk, C.site(k), wk, Y, tk, false);
+        __syncthreads();
+        T *d = dst + (size_t)ch * tn * wn;
+        gemm<true, false, T>(tn, wn, tk, Tk, tk, Y, tk, d, tn, false);
+        __syncthreads();
+        gemm<true, false, T>(tn, wn, tk, Tk + (size_t)tk * tn, tk, Y + (size_t)tk * wn, tk, d, tn, true);
+        __syncthreads();
+    }
+}
+
+// Two-site mixed tensor M[(s'+2t')*4 + (s+2t)] = <G(s',s), V(t',t)> at a bond
+// with trial dim tb and walker dim wb.  g[ch], ch = 2 s' + s, with leading dim gld.
+template <typename T>
+__device__ void pair_tensor(const Ctx<T> &C, int tb, int wb, const T *const g[4], int gld, double *M) {
+    const T *V = C.ts + C.sl.V;
+    double acc[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc[q] = 0.0;
+    const int nel = tb * wb;
+    for (int e = threadIdx.x; e < nel; e += MQE_NT) {
+        const int a = e % tb, b = e / tb;
+        double gv[4], vv[4];
+#pragma unroll
+        for (int ch = 0; ch < 4; ++ch) gv[ch] = (double)g[ch][a + (size_t)gld * b];
+#pragma unroll
+        for (int ch = 0; ch < 4; ++ch) {
+            const int tp = ch >> 1, t = ch & 1;
+            vv[ch] = (double)V[(size_t)tp * tb * 2 * wb + (size_t)t * tb * wb + a + (size_t)tb * b];
+        }
+#pragma unroll
+        for (int sp = 0; sp < 2; ++sp)
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int tp = 0; tp < 2; ++tp)
+#pragma unroll
+                    for (int t = 0; t < 2; ++t) acc[(sp + 2 * tp) * 4 + (s + 2 * t)] += gv[2 * sp + s] * vv[2 * tp + t];
+    }
+    block_sum_n<16>(acc, C.red);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) M[q] = acc[q];
+    }
+    __syncthreads();
+}
+
+// Mixed tensor for the pair (i, j) from L_i, sites i..j and R_{j+1}.
+template <typename T>
+__device__ void pair_for_coupling(const Ctx<T> &C, int i, int j, double *M) {
+    compute_G(C, i, C.Lk(i));
+    const int tn = C.tv[i + 1], wn = C.wv[i + 1];
+    const T *G = C.ts + C.sl.G;
+    const T *g[4];
+    int gld;
+    if (j == i + 1) {
+        for (int ch = 0; ch < 4; ++ch) {
+            const int sp = ch >> 1, s = ch & 1;
+            g[ch] = G + (size_t)sp * tn + (size_t)s * wn * 2 * tn;
+        }
+        gld = 2 * tn;
+    } else {
+        // copy the four blocks into contiguous channels, then carry to bond j
+        T *A = C.ts + C.sl.GcA, *B = C.ts + C.sl.GcB;
+        for (int e = threadIdx.x; e < 4 * tn * wn; e += MQE_NT) {
+            const int ch = e / (tn * wn), r = e % (tn * wn), a = r % tn, b = r / tn;
+            const int sp = ch >> 1, s = ch & 1;
+            A[e] = G[(size_t)sp * tn + (size_t)s * wn * 2 * tn + a + (size_t)2 * tn * b];
+        }
+        __syncthreads();
+        for (int k = i + 1; k < j; ++k) {
+            carry_channels(C, k, A, B);
+            T *t = A; A = B; B = t;
+        }
+        const int tb = C.tv[j], wb = C.wv[j];
+        for (int ch = 0; ch < 4; ++ch) g[ch] = A + (size_t)ch * tb * wb;
+        gld = tb;
+    }
+    compute_V(C, j);
+    pair_tensor(C, C.tv[j], C.wv[j], g, gld, M);
+}
+
+// L_{k+1} = sum_s T_k[s]^T L_k phi_k[s]
+template <typename T>
+__device__ void advance_L(const Ctx<T> &C, int k) {
+    const int tk = C.tv[k], tn = C.tv[k + 1], wk = C.wv[k], wn = C.wv[k + 1];
+    T *Y = C.ts + C.sl.Y;
+    gemm<false, false, T>(tk, 2 * wn, wk, C.Lk(k), tk, C.site(k), wk, Y, tk, false);
+    __syncthreads();
+    const T *Tk = C.tsite(k);
+    T *Ln = C.Lk(k + 1);
+    gemm<true, false, T>(tn, wn, tk, Tk, tk, Y, tk, Ln, tn, false);
+    __syncthreads();
+    gemm<true, false, T>(tn, wn, tk, Tk + (size_t)tk * tn, tk, Y + (size_t)tk * wn, tk, Ln, tn, true);
+    __syncthreads();
+}
+
+// Nn = sum_s phi_k[s]^T N phi_k[s]  (w_{k+1} x w_{k+1})
+template <typename T>
+__device__ void advance_N(const Ctx<T> &C, int k, const T *N, T *Nn) {
+    const int wk = C.wv[k], wn = C.wv[k + 1];
+    T *X = C.ts + C.sl.Y;
+    const T *ph = C.site(k);
+    gemm<false, false, T>(wk, 2 * wn, wk, N, wk, ph, wk, X, wk, false);
+    __syncthreads();
+    gemm<true, false, T>(wn, wn, wk, ph, wk, X, wk, Nn, wn, false);
+    __syncthreads();
+    gemm<true, false, T>(wn, wn, wk, ph + (size_t)wk * wn, wk, X + (size_t)wk * wn, wk, Nn, wn, true);
+    __syncthreads();
+}
+
+// <phi_k, N phi_k> with the sites right of k right-orthonormal = squared norm.
+template <typename T>
+__device__ double norm2_at(const Ctx<T> &C, int k, const T *N) {
+    const int wk = C.wv[k], wn = C.wv[k + 1];
+    T *Z = C.ts + C.sl.Y;
+    const T *ph = C.site(k);
+    gemm<false, false, T>(wk, 2 * wn, wk, N, wk, ph, wk, Z, wk, false);
+    __syncthreads();
+    double s = 0.0;
+    for (int e = threadIdx.x; e < wk * 2 * wn; e += MQE_NT) s += (double)ph[e] * (double)Z[e];
+    return block_sum(s, C.red);
+}
+
+template <typename T>
+__device__ void scale_site(const Ctx<T> &C, int k, double f) {
+    T *ph = C.site(k);
+    const int n = C.wv[k] * C.wv[k + 1] * 2;
+    for (int e = threadIdx.x; e < n; e += MQE_NT) ph[e] = (T)((double)ph[e] * f);
+    __syncthreads();
+}
+
+// phi_k[s] <- sum_s0 O[s + 2 s0] phi_k[s0]
+template <typename T>
+__device__ void apply_op(const Ctx<T> &C, int k, const double *O) {
+    T *ph = C.site(k);
+    const int blk = C.wv[k] * C.wv[k + 1];
+    const T o00 = (T)O[0], o10 = (T)O[1], o01 = (T)O[2], o11 = (T)O[3];
+    for (int e = threadIdx.x; e < blk; e += MQE_NT) {
+        const T a0 = ph[e], a1 = ph[e + blk];
+        ph[e] = o00 * a0 + o01 * a1;
+        ph[e + blk] = o10 * a0 + o11 * a1;
+    }
+    __syncthreads();
+}
+
+// LQ of site k (w_k x 2 w_{k+1}) in FP64: phi_k <- Q (right-orthonormal),
+// phi_{k-1} <- phi_{k-1} Lmat, then R_k = sum_t T_k[t] R_{k+1} Q[t]^T.
+// Leaves V (all four blocks) computed for bond k.
+template <typename T>
+__device__ void lq_site(const Ctx<T> &C, int k) {
+    const int wk = C.wv[k], wn = C.wv[k + 1], wp = C.wv[k - 1];
+    const int m = 2 * wn, n = wk;
+    double *A = C.ds + C.sl.AQ, *R = C.ds + C.sl.RQ;
+    T *ph = C.site(k);
+    for (int e = threadIdx.x; e < m * n; e += MQE_NT) {
+        const int r = e % m, c = e / m;
+        A[e] = (double)ph[c + (size_t)wk * r];
+    }
+    __syncthreads();
+    householder_qr(m, n, A, R, C.ds + C.sl.TAU, C.ds + C.sl.PARTS, C.ds + C.sl.DOTS, C.red);
+    for (int e = threadIdx.x; e < m * n; e += MQE_NT) {
+        const int r = e % m, c = e / m;
+        ph[c + (size_t)wk * r] = (T)A[e];
+    }
+    T *LM = C.ts + C.sl.LM;
+    for (int e = threadIdx.x; e < n * n; e += MQE_NT) {
+        const int a = e % n, b = e / n;
+        LM[e] = (T)R[b + (size_t)n * a];
+    }
+    __syncthreads();
+    // push Lmat into the left neighbour
+    T *pl = C.site(k - 1), *tmp = C.ts + C.sl.TMP;
+    for (int s = 0; s < 2; ++s)
+        gemm<false, false, T>(wp, wk, wk, pl + (size_t)s * wp * wk, wp, LM, wk, tmp + (size_t)s * wp * wk, wp, false);
+    __syncthreads();
+    for (int e = threadIdx.x; e < 2 * wp * wk; e += MQE_NT) pl[e] = tmp[e];
+    __syncthreads();
+    // right environment of bond k
+    compute_V(C, k);
+    const int tk = C.tv[k];
+    const T *V = C.ts + C.sl.V;
+    T *Rk = C.Rk(k);
+    for (int e = threadIdx.x; e < tk * wk; e += MQE_NT) Rk[e] = V[e] + V[(size_t)tk * 2 * wk + (size_t)tk * wk + e];
+    __syncthreads();
+}
+
+// <trial|phi> and sum_s sz[s] <trial|S^z_0|phi> at site 0 (L_0 = 1, right part via R_1).
+template <typename T>
+__device__ void site0_overlap(const Ctx<T> &C, double *ov, double *szv) {
+    const int t1 = C.tv[1], w1 = C.wv[1];
+    const T *T0 = C.tsite(0), *p0 = C.site(0), *R1 = C.Rk(1);
+    double a[2] = {0.0, 0.0};
+    for (int e = threadIdx.x; e < t1 * w1; e += MQE_NT) {
+        const int al = e % t1, be = e / t1;
+        const double r = (double)R1[al + (size_t)t1 * be];
+        for (int s = 0; s < 2; ++s) a[s] += (double)T0[al + (size_t)t1 * s] * (double)p0[be + (size_t)w1 * s] * r;
+    }
+    block_sum_n<2>(a, C.red);
+    *ov = a[0] + a[1];
+    *szv = C.S->Sz[0] * a[0] + C.S->Sz[1] * a[1];
+}
+
+template <typename T>
+__device__ void set_one(T *p) {
+    if (threadIdx.x == 0) p[0] = (T)1;
+    __syncthreads();
+}
+
+// Right-canonicalise the whole walker with R environments, normalise at site 0.
+// Returns the (sign-fixed) overlap of the normalised state.
+template <typename T>
+__device__ double right_canonicalize(const Ctx<T> &C) {
+    const int L = C.S->L;
+    set_one(C.Rk(L));
+    for (int k = L - 1; k >= 1; --k) lq_site(C, k);
+    double ov, szv;
+    site0_overlap(C, &ov, &szv);
+    double nrm2 = 0.0;
+    {
+        const T *p0 = C.site(0);
+        double s = 0.0;
+        for (int e = threadIdx.x; e < 2 * C.wv[1]; e += MQE_NT) s += (double)p0[e] * (double)p0[e];
+        nrm2 = block_sum(s, C.red);
+    }
+    const double nrm = sqrt(nrm2);
+    double O = ov / nrm;
+    double f = 1.0 / nrm;
+    if (O < 0.0) { O = -O; f = -f; }
+    scale_site(C, 0, f);
+    return O;
+}
+
+// Energy sweep from the right end (L_k valid for all k), see file header.
+// Returns e_tot = <trial|H|phi> of the state as it was on entry and writes the
+// overlap of the normalised state to *O_out, <trial|phi> before normalisation to *Ou.
+template <typename T>
+__device__ double energy_sweep(const Ctx<T> &C, double *O_out, double *Ou) {
+    const Sys &S = *C.S;
+    const int L = S.L;
+    __shared__ double M[16];
+    set_one(C.Rk(L));
+    double etot = 0.0;
+    for (int k = L - 1; k >= 1; --k) {
+        lq_site(C, k);  // V of bond k now holds T_k R_{k+1} Q_k^T blocks
+        for (int q = S.cs_ptr[k]; q < S.cs_ptr[k + 1]; ++q) {
+            const int c = S.cs_idx[q], i = S.first[c];
+            compute_G(C, i, C.Lk(i));
+            const int tn = C.tv[i + 1], wn = C.wv[i + 1];
+            const T *G = C.ts + C.sl.G;
+            const T *g[4];
+            int gld;
+            if (i == k - 1) {
+                for (int ch = 0; ch < 4; ++ch) {
+                    const int sp = ch >> 1, s = ch & 1;
+                    g[ch] = G + (size_t)sp * tn + (size_t)s * wn * 2 * tn;
+                }
+                gld = 2 * tn;
+            } else {
+                T *A = C.ts + C.sl.GcA, *B = C.ts + C.sl.GcB;
+                for (int e = threadIdx.x; e < 4 * tn * wn; e += MQE_NT) {
+                    const int ch = e / (tn * wn), r = e % (tn * wn), a = r % tn, b = r / tn;
+                    const int sp = ch >> 1, s = ch & 1;
+                    A[e] = G[(size_t)sp * tn + (size_t)s * wn * 2 * tn + a + (size_t)2 * tn * b];
+                }
+                __syncthreads();
+                for (int kk = i + 1; kk < k; ++kk) {
+                    carry_channels(C, kk, A, B);
+                    T *t = A; A = B; B = t;
+                }
+                for (int ch = 0; ch < 4; ++ch) g[ch] = A + (size_t)ch * C.tv[k] * C.wv[k];
+                gld = C.tv[k];
+            }
+            pair_tensor(C, C.tv[k], C.wv[k], g, gld, M);
+            double e = 0.0;
+            for (int q2 = 0; q2 < 16; ++q2) e += S.SS[q2] * M[q2];
+            etot += S.Jc[c] * e;
+        }
+        if (S.h != 0.0) {
+            // <trial|S^z_k|phi> = sum_t sz[t] <L'_k, V(t,t)>, L'_k the current-gauge left env
+            compute_G(C, k - 1, C.Lk(k - 1));
+            const int tk = C.tv[k], wk = C.wv[k];
+            const T *G = C.ts + C.sl.G, *V = C.ts + C.sl.V;
+            double a[2] = {0.0, 0.0};
+            for (int e = threadIdx.x; e < tk * wk; e += MQE_NT) {
+                const int al = e % tk, be = e / tk;
+                const double lp = (double)G[al + (size_t)2 * tk * be] +
+                                  (double)G[tk + (size_t)wk * 2 * tk + al + (size_t)2 * tk * be];
+                a[0] += lp * (double)V[al + (size_t)tk * be];
+                a[1] += lp * (double)V[(size_t)tk * 2 * wk + (size_t)tk * wk + al + (size_t)tk * be];
+            }
+            block_sum_n<2>(a, C.red);
+            etot -= S.h * (S.Sz[0] * a[0] + S.Sz[1] * a[1]);
+        }
+    }
+    double ov, szv;
+    site0_overlap(C, &ov, &szv);
+    if (S.h != 0.0) etot -= S.h * szv;
+    double nrm2;
+    {
+        const T *p0 = C.site(0);
+        double s = 0.0;
+        for (int e = threadIdx.x; e < 2 * C.wv[1]; e += MQE_NT) s += (double)p0[e] * (double)p0[e];
+        nrm2 = block_sum(s, C.red);
+    }
+    const double nrm = sqrt(nrm2);
+    double O = ov / nrm, f = 1.0 / nrm;
+    if (O < 0.0) { O = -O; f = -f; }
+    scale_site(C, 0, f);
+    *O_out = O;
+    *Ou = ov;
+    return etot;
+}
+
+template <typename T>
+__device__ void apply_field_everywhere(const Ctx<T> &C) {
+    const double O[4] = {C.S->ssp[0], 0.0, 0.0, C.S->ssp[1]};
+    for (int k = 0; k < C.S->L; ++k) apply_op(C, k, O);
+}
+
+template <typename T>
+__global__ __launch_bounds__(MQE_NT) void walker_kernel(Sys S, Pool P, const T *trial, int flags) {
+    const int b = blockIdx.x;
+    const int n = *P.n;
+    if (b >= n) return;
+    const int slot = P.order[b];
+    const int L = S.L;
+
+    __shared__ int tv[MQE_MAXL + 1], wv[MQE_MAXL + 1];
+    __shared__ double red[MQE_NT + 16 * MQE_WAVES];
+    __shared__ double M[16], comb[16], Ux[4], Vx[4];
+    __shared__ double sh_scal[4];
+    __shared__ int sh_g;
+
+    for (int k = threadIdx.x; k <= L; k += MQE_NT) {
+        tv[k] = S.tvd[k];
+        wv[k] = P.wvd[(size_t)slot * (L + 1) + k];
+    }
+    __syncthreads();
+
+    Ctx<T> C;
+    C.S = &S;
+    C.trial = trial;
+    C.phi = (T *)P.phi + (size_t)slot * P.wflat;
+    C.Lenv = (T *)P.Lenv + (size_t)slot * P.eflat;
+    C.Renv = (T *)P.Renv + (size_t)slot * P.eflat;
+    C.woff = P.woff;
+    C.eoff = P.eoff;
+    C.tv = tv;
+    C.wv = wv;
+    C.sl = scratch_layout(S.Dt, S.Dw, S.Ng);
+    char *base = P.scratch + (size_t)b * P.scratch_bytes;
+    C.ts = (T *)base;
+    long tb = (C.sl.t_total * (long)sizeof(T) + 255) & ~255L;
+    C.ds = (double *)(base + tb);
+    C.red = red;
+
+    double w = P.w[slot];
+    double O = P.O[slot];
+
+    set_one(C.Lk(0));
+
+    if (flags & F_PREP) {
+        for (int k = 0; k < L - 1; ++k) advance_L(C, k);
+        double On, Ou;
+        const double e = energy_sweep(C, &On, &Ou);
+        if (threadIdx.x == 0) {
+            P.EL[slot] = e / O;  // stored overlap, MPSQMC2.cpp:447 / Walker.cpp:72-77
+            P.O[slot] = On;
+        }
+        return;
+    }
+
+    // ---------------- propagation (MPSQMC2.cpp:532-600) ----------------
+    if (S.h != 0.0) {
+        apply_field_everywhere(C);
+        O = right_canonicalize(C);
+    }
+    T *Ncur = C.ts + C.sl.N0, *Nnxt = C.ts + C.sl.N1;
+    set_one(Ncur);
+    int p = 0;
+    for (int c = 0; c < S.nC; ++c) {
+        const int i = S.first[c], j = S.second[c];
+        while (p < i) {
+            advance_L(C, p);
+            advance_N(C, p, Ncur, Nnxt);
+            T *t = Ncur; Ncur = Nnxt; Nnxt = t;
+            ++p;
+        }
+        pair_for_coupling(C, i, j, M);
+        // combos[a + 4b] = sqrt(s_a s_b) <trial| U_a (x) V_b |phi>  (MPSQMC2.cpp:540-549)
+        const double *tS = S.tS + 4 * c, *tU = S.tU + 16 * c, *tVT = S.tVT + 16 * c;
+        if (threadIdx.x < 16) {
+            const int a = threadIdx.x & 3, bb = threadIdx.x >> 2;
+            double s = 0.0;
+            for (int sp = 0; sp < 2; ++sp)
+                for (int ss = 0; ss < 2; ++ss) {
+                    const double ua = tU[sp + 2 * (ss + 2 * a)];
+                    if (ua == 0.0) continue;
+                    for (int tp = 0; tp < 2; ++tp)
+                        for (int t = 0; t < 2; ++t)
+                            s += ua * tVT[bb + 4 * (tp + 2 * t)] * M[(sp + 2 * tp) * 4 + (ss + 2 * t)];
+                }
+            comb[threadIdx.x] = sqrt(tS[a] * tS[bb]) * s;
+        }
+        __syncthreads();
+        // PDF over the grid (MPSQMC2.cpp:552-574)
+        double *pdf = C.ds + C.sl.PDF, *cum = C.ds + C.sl.CUM;
+        const int Ng = S.Ng;
+        const double pref = 4.0 / (O * Ng);
+        double part = 0.0;
+        for (int g = threadIdx.x; g < Ng; g += MQE_NT) {
+            const double *x = S.grid + 4 * g;
+            double f = 0.0;
+            for (int l = 0; l < 4; ++l)
+                for (int r = 0; r < 4; ++r) f += comb[l + 4 * r] * x[l] * x[r];
+            double v = f > 0.0 ? f : 0.0;
+            v *= pref;
+            pdf[g] = v;
+            part += v;
+        }
+        const double pdfnorm = block_sum(part, red);
+        const double inv = 1.0 / pdfnorm;
+        for (int g = threadIdx.x; g < Ng; g += MQE_NT) pdf[g] *= inv;
+        __syncthreads();
+        block_inclusive_scan(Ng, pdf, cum, red);
+        const double u = P.u ? P.u[(size_t)b * S.nC + c] : philox_uniform(P.seed, (uint32_t)b, (uint32_t)c, (uint32_t)P.step, 0u);
+        double cnt = 0.0;
+        for (int g = threadIdx.x; g < Ng - 1; g += MQE_NT) cnt += (cum[g] <= u) ? 1.0 : 0.0;
+        cnt = block_sum(cnt, red);
+        const int gs = (int)cnt;
+        w *= pdfnorm;
+        if (threadIdx.x == 0) {
+            sh_g = gs;
+            if (P.idx) P.idx[(size_t)b * S.nC + c] = gs;
+            if (!(pdfnorm > 0.0) || !isfinite(pdfnorm)) atomicOr(P.status, 1);
+            // sampled operators (MPSstate.cpp:589-631)
+            const double *x = S.grid + 4 * gs;
+            double f = 0.0;
+            for (int l = 0; l < 4; ++l)
+                for (int r = 0; r < 4; ++r) f += comb[l + 4 * r] * x[l] * x[r];
+            sh_scal[0] = f;
+            for (int q = 0; q < 4; ++q) { Ux[q] = 0.0; Vx[q] = 0.0; }
+            for (int k = 0; k < 4; ++k) {
+                const double pf = sqrt(tS[k] * 4.0) * x[k];
+                if (pf == 0.0) continue;
+                for (int up = 0; up < 2; ++up)
+                    for (int dn = 0; dn < 2; ++dn) {
+                        Ux[up + 2 * dn] += pf * tU[up + 2 * (dn + 2 * k)];
+                        Vx[up + 2 * dn] += pf * tVT[k + 4 * (up + 2 * dn)];
+                    }
+            }
+        }
+        __syncthreads();
+        const double fstar = sh_scal[0];
+        apply_op(C, i, Ux);
+        apply_op(C, j, Vx);
+        const int nf = S.nextfirst[c];
+        double nrm2;
+        if (j == i + 1 && nf > i) {
+            // nearest-neighbour fast path: boundary moves to j
+            const int tn = tv[i + 1], wn = wv[i + 1];
+            const T *G = C.ts + C.sl.G;
+            T *Ln = C.Lk(i + 1);
+            const T u00 = (T)Ux[0], u10 = (T)Ux[1], u01 = (T)Ux[2], u11 = (T)Ux[3];
+            for (int e = threadIdx.x; e < tn * wn; e += MQE_NT) {
+                const int a = e % tn, bq = e / tn;
+                const size_t o = a + (size_t)2 * tn * bq;
+                // L_{i+1} = sum_{s,s0} Ux[s,s0] G(s, s0)
+                const T g00 = G[o], g01 = G[o + (size_t)wn * 2 * tn], g10 = G[o + tn],
+                        g11 = G[o + tn + (size_t)wn * 2 * tn];
+                Ln[a + (size_t)tn * bq] = u00 * g00 + u01 * g01 + u10 * g10 + u11 * g11;
+            }
+            __syncthreads();
+            advance_N(C, i, Ncur, Nnxt);
+            T *t = Ncur; Ncur = Nnxt; Nnxt = t;
+            p = i + 1;
+            nrm2 = norm2_at(C, j, Ncur);
+            const double nrm = sqrt(nrm2);
+            scale_site(C, j, 1.0 / nrm);
+            O = 4.0 * fstar / nrm;
+        } else {
+            for (int k = j; k >= i + 1; --k) lq_site(C, k);
+            nrm2 = norm2_at(C, i, Ncur);
+            const double nrm = sqrt(nrm2);
+            scale_site(C, i, 1.0 / nrm);
+            O = 4.0 * fstar / nrm;
+        }
+    }
+    while (p < L - 1) {
+        advance_L(C, p);
+        advance_N(C, p, Ncur, Nnxt);
+        T *t = Ncur; Ncur = Nnxt; Nnxt = t;
+        ++p;
+    }
+    if (S.h != 0.0) {
+        apply_field_everywhere(C);
+        set_one(Ncur);
+        for (int k = 0; k < L - 1; ++k) {
+            advance_L(C, k);
+            advance_N(C, k, Ncur, Nnxt);
+            T *t = Ncur; Ncur = Nnxt; Nnxt = t;
+        }
+        const double nrm = sqrt(norm2_at(C, L - 1, Ncur));
+        scale_site(C, L - 1, 1.0 / nrm);
+    }
+    // ---------------- local energy (Walker.cpp:72-77) ----------------
+    double On, Ou;
+    const double e = energy_sweep(C, &On, &Ou);
+    if (threadIdx.x == 0) {
+        P.w[slot] = w;
+        P.O[slot] = On;
+        P.EL[slot] = e / Ou;
+        if (!isfinite(w)) atomicOr(P.status, 2);
+    }
+}
+
+// out[b] = <other | walker_b> (transfer matrices, MPSstate.cpp:248-284).
+template <typename T>
+__global__ __launch_bounds__(MQE_NT) void overlap_kernel(int L, const int *ovd, const long *ooff, const T *other,
+                                                       Pool P, int Dmax, double *out) {
+    const int b = blockIdx.x;
+    if (b >= *P.n) return;
+    const int slot = P.order[b];
+    __shared__ int ov[MQE_MAXL + 1], wv[MQE_MAXL + 1];
+    for (int k = threadIdx.x; k <= L; k += MQE_NT) {
+        ov[k] = ovd[k];
+        wv[k] = P.wvd[(size_t)slot * (L + 1) + k];
+    }
+    __syncthreads();
+    const T *phi = (const T *)P.phi + (size_t)slot * P.wflat;
+    T *E = (T *)(P.scratch + (size_t)b * P.scratch_bytes);
+    T *E2 = E + (size_t)Dmax * Dmax;
+    T *Y = E2 + (size_t)Dmax * Dmax;
+    set_one(E);
+    for (int k = 0; k < L; ++k) {
+        const int oa = ov[k], ob = ov[k + 1], wa = wv[k], wb = wv[k + 1];
+        const T *A = other + ooff[k], *Bt = phi + P.woff[k];
+        // Y = E A (wa x 2 ob) with E (wa x oa)
+        gemm<false, false, T>(wa, 2 * ob, oa, E, wa, A, oa, Y, wa, false);
+        __syncthreads();
+        // E2 = sum_s B[s]^T Y[s]  (wb x ob)
+        gemm<true, false, T>(wb, ob, wa, Bt, wa, Y, wa, E2, wb, false);
+        __syncthreads();
+        gemm<true, false, T>(wb, ob, wa, Bt + (size_t)wa * wb, wa, Y + (size_t)wa * ob, wa, E2, wb, true);
+        __syncthreads();
+        T *t = E; E = E2; E2 = t;
+    }
+    if (threadIdx.x == 0) out[b] = (double)E[0];
+}
+
+// ---------------------------------------------------------------------------
+// reductions, bookkeeping and population control (single workgroup, 1024 threads)
+// ---------------------------------------------------------------------------
+#define RED_NT 1024
+
+__device__ double rsum(double v, double *sh) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int q = 0; q < RED_NT / 64; ++q) t += sh[q];
+    __syncthreads();
+    return t;
+}
+
+// exclusive scan of ints over RED_NT threads (one value per thread); returns the total.
+__device__ int rscan_excl(int v, int *out_excl, int *sh) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int s = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        int o = __shfl_up(s, off, 64);
+        if (lane >= off) s += o;
+    }
+    __syncthreads();
+    if (lane == 63) sh[wid] = s;
+    __syncthreads();
+    int base = 0, tot = 0;
+    for (int q = 0; q < RED_NT / 64; ++q) {
+        if (q < wid) base += sh[q];
+        tot += sh[q];
+    }
+    *out_excl = base + s - v;
+    __syncthreads();
+    return tot;
+}
+
+struct Book {
+    const int *n;
+    int *order, *order2;
+    double *w, *O, *eh, *EL;
+    double *sums;     // [0] sum w  [1] scaling  [2] E_T  [3] E_P  [4] Omega  [5] N
+    double *trace;    // 6 doubles per step
+    int *ncopy, *newpos, *used, *freelist, *copy_src, *copy_dst, *n_extra, *n_new, *n_used;
+    int *status;
+    int cap, n_desired;
+    double dtau;
+};
+
+// sum of weights; if device_scaling also scaling and E_T (MPSQMC2.cpp:480-484)
+__global__ __launch_bounds__(RED_NT) void sumw_kernel(Book B) {
+    __shared__ double sh[RED_NT / 64];
+    const int n = *B.n;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < n; b += RED_NT) s += B.w[B.order[b]];
+    s = rsum(s, sh);
+    if (threadIdx.x == 0) {
+        B.sums[0] = s;
+        const double avg = s / n;
+        const double scaling = B.n_desired / (n * avg);
+        B.sums[1] = scaling;
+        B.sums[2] = log(scaling) / B.dtau;
+        B.sums[5] = n;
+    }
+}
+
+// EnergyFunctionAndHistory (MPSQMC2.cpp:688-747)
+__global__ __launch_bounds__(RED_NT) void energy_kernel(Book B, int step, int dofluct, int trace_row) {
+    __shared__ double sh[RED_NT / 64];
+    const int n = *B.n;
+    double num = 0.0, den = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int b = threadIdx.x; b < n; b += RED_NT) {
+        const int sl = B.order[b];
+        const double e = B.EL[sl], w = B.w[sl];
+        num += w * e;
+        den += w;
+        if (dofluct) {
+            const double h = B.eh[sl] + e;
+            B.eh[sl] = h;
+            const double a = h / step;
+            s1 += a;
+            s2 += a * a;
+        }
+    }
+    num = rsum(num, sh);
+    den = rsum(den, sh);
+    s1 = rsum(s1, sh);
+    s2 = rsum(s2, sh);
+    if (threadIdx.x == 0) {
+        B.sums[3] = num / den;
+        B.sums[4] = dofluct ? (s2 - s1 * s1 / n) / n : 0.0;
+        if (trace_row >= 0) {
+            double *r = B.trace + 6L * trace_row;
+            r[0] = step; r[1] = n; r[2] = B.sums[3]; r[3] = B.sums[2]; r[4] = B.sums[4]; r[5] = B.sums[0];
+        }
+    }
+}
+
+// SeparatePopulationControl (MPSQMC2.cpp:613-686): copy counts, new order,
+// slot assignment for the extra copies.  scaling from host (>0) or B.sums[1].
+__global__ __launch_bounds__(RED_NT) void popcontrol_kernel(Book B, double host_scaling, const double *u, int nu,
+                                                            unsigned long long seed, int step) {
+    __shared__ int shi[RED_NT / 64];
+    __shared__ int carry_draw, carry_pos, carry_extra, carry_free;
+    const int n = *B.n;
+    const double scaling = host_scaling > 0.0 ? host_scaling : B.sums[1];
+    if (threadIdx.x == 0) { carry_draw = 0; carry_pos = 0; carry_extra = 0; carry_free = 0; }
+    for (int s = threadIdx.x; s < B.cap; s += RED_NT) B.used[s] = 0;
+    __syncthreads();
+    // pass 1: copy counts in population order, draws consumed in order
+    for (int base = 0; base < n; base += RED_NT) {
+        const int b = base + threadIdx.x;
+        double sw = 0.0;
+        int need = 0;
+        if (b < n) {
+            sw = B.w[B.order[b]] * scaling;
+            need = (sw < 0.25 || sw > 1.5) ? 1 : 0;
+        }
+        int ex;
+        const int tot = rscan_excl(need, &ex, shi);
+        int c = 1;
+        double nw = sw;
+        if (b < n) {
+            if (need) {
+                const int k = carry_draw + ex;
+                const double r = u ? (k < nu ? u[k] : 0.0)
+                                   : philox_uniform(seed, (uint32_t)b, 0xFFFFFFFFu, (uint32_t)step, 1u);
+                c = (int)(sw + r);
+                nw = (sw < 0.25) ? 1.0 : sw / c;
+            }
+            B.ncopy[b] = c;
+            B.w[B.order[b]] = nw;  // applied to survivors; copies read it before the copy kernel
+        }
+        int pos;
+        const int totc = rscan_excl(b < n ? c : 0, &pos, shi);
+        if (b < n) B.newpos[b] = carry_pos + pos;
+        __syncthreads();
+        if (threadIdx.x == 0) { carry_draw += tot; carry_pos += totc; }
+        __syncthreads();
+    }
+    const int total = carry_pos;
+    if (threadIdx.x == 0) { *B.n_used = carry_draw; *B.n_new = total; }
+    if (total > B.cap) {
+        if (threadIdx.x == 0) { atomicOr(B.status, 4); *B.n_extra = 0; }
+        return;
+    }
+    // survivors keep their slot
+    for (int b = threadIdx.x; b < n; b += RED_NT)
+        if (B.ncopy[b] > 0) B.used[B.order[b]] = 1;
+    __syncthreads();
+    // free list (slots not used by survivors), in slot order
+    for (int base = 0; base < B.cap; base += RED_NT) {
+        const int s = base + threadIdx.x;
+        const int f = (s < B.cap && !B.used[s]) ? 1 : 0;
+        int ex;
+        const int tot = rscan_excl(f, &ex, shi);
+        if (f) B.freelist[carry_free + ex] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) carry_free += tot;
+        __syncthreads();
+    }
+    // new order and copy list
+    for (int base = 0; base < n; base += RED_NT) {
+        const int b = base + threadIdx.x;
+        const int c = (b < n) ? B.ncopy[b] : 0;
+        int ex;
+        const int tot = rscan_excl(c > 1 ? c - 1 : 0, &ex, shi);
+        if (b < n && c > 0) {
+            const int pos = B.newpos[b];
+            const int slot = B.order[b];
+            B.order2[pos] = slot;
+            for (int q = 1; q < c; ++q) {
+                const int e = carry_extra + ex + q - 1;
+                const int dst = B.freelist[e];
+                B.order2[pos + q] = dst;
+                B.copy_src[e] = slot;
+                B.copy_dst[e] = dst;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) carry_extra += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *B.n_extra = carry_extra;
+}
+
+// Copy walker data for the extra copies; then install the new order.
+template <typename T>
+__global__ __launch_bounds__(MQE_NT) void copy_walkers_kernel(Book B, Pool P, int L) {
+    const int e = blockIdx.x;
+    if (e >= *B.n_extra) return;
+    const int src = B.copy_src[e], dst = B.copy_dst[e];
+    const T *ps = (const T *)P.phi + (size_t)src * P.wflat;
+    T *pd = (T *)P.phi + (size_t)dst * P.wflat;
+    for (long q = threadIdx.x; q < P.wflat; q += MQE_NT) pd[q] = ps[q];
+    const T *rs = (const T *)P.Renv + (size_t)src * P.eflat;
+    T *rd = (T *)P.Renv + (size_t)dst * P.eflat;
+    for (long q = threadIdx.x; q < P.eflat; q += MQE_NT) rd[q] = rs[q];
+    for (int k = threadIdx.x; k <= L; k += MQE_NT) P.wvd[(size_t)dst * (L + 1) + k] = P.wvd[(size_t)src * (L + 1) + k];
+    if (threadIdx.x == 0) {
+        P.w[dst] = P.w[src];
+        P.O[dst] = P.O[src];
+        P.eh[dst] = P.eh[src];
+        P.EL[dst] = P.EL[src];
+    }
+}
+
+__global__ void install_order_kernel(Book B) {
+    const int total = *B.n_new;
+    if (total > B.cap) return;
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < total; b += gridDim.x * blockDim.x) B.order[b] = B.order2[b];
+}
+
+__global__ void set_n_kernel(Book B, int *n) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const int t = *B.n_new;
+        if (t <= B.cap) *n = t;
+    }
+}
+
+}  // namespace mqe
+
+// ===========================================================================
+// host engine
+// ===========================================================================
+using namespace mqe;
+
+#define HIPCHK(x)                                                                            \
+    do {                                                                                     \
+        hipError_t _e = (x);                                                                 \
+        if (_e != hipSuccess) {                                                              \
+            char _b[256];                                                                    \
+            snprintf(_b, sizeof(_b), "HIP error %s at %s:%d", hipGetErrorString(_e), __FILE__, \
+                     __LINE__);                                                              \
+            set_error(_b);                                                                   \
+            return MQE_EDEVICE;                                                              \
+        }                                                                                    \
+    } while (0)
+
+struct mqe_engine {
+    mqe_params p;
+    int L = 0, nC = 0, Ng = 0, gdim = 0, Dt = 0, Dw = 0;
+    bool fp32 = false;
+    hipStream_t stream = nullptr;
+    std::vector<int> tvd, wmax;
+    std::vector<long> toff, woff, eoff;
+    long wflat = 0, eflat = 0, scratch_bytes = 0;
+    // device
+    void *d_trial = nullptr;
+    int *d_tvd = nullptr, *d_first = nullptr, *d_second = nullptr, *d_nextfirst = nullptr, *d_csptr = nullptr,
+        *d_csidx = nullptr;
+    long *d_toff = nullptr, *d_woff = nullptr, *d_eoff = nullptr;
+    double *d_Jc = nullptr, *d_S = nullptr, *d_U = nullptr, *d_VT = nullptr, *d_grid = nullptr;
+    void *d_phi = nullptr, *d_Renv = nullptr, *d_Lenv = nullptr;
+    int *d_wvd = nullptr;
+    double *d_w = nullptr, *d_O = nullptr, *d_eh = nullptr, *d_EL = nullptr;
+    int *d_order = nullptr, *d_order2 = nullptr, *d_n = nullptr, *d_idx = nullptr;
+    double *d_u = nullptr;
+    long u_cap = 0;
+    char *d_scratch = nullptr;
+    int *d_status = nullptr;
+    double *d_sums = nullptr, *d_trace = nullptr;
+    int trace_cap = 0;
+    int *d_ncopy = nullptr, *d_newpos = nullptr, *d_used = nullptr, *d_free = nullptr, *d_csrc = nullptr,
+        *d_cdst = nullptr, *d_nextra = nullptr, *d_nnew = nullptr, *d_nused = nullptr;
+    double *d_tmpout = nullptr;
+    int n_host = 0;
+    bool loaded = false, prepared = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_ms = 0.f;
+    int last_walkers = 0;
+    hipGraphExec_t graph_exec = nullptr;
+    Sys sys;
+};
+
+static Pool make_pool(mqe_engine *e, const double *u, int step) {
+    Pool P;
+    P.phi = e->d_phi;
+    P.Renv = e->d_Renv;
+    P.Lenv = e->d_Lenv;
+    P.wvd = e->d_wvd;
+    P.w = e->d_w;
+    P.O = e->d_O;
+    P.eh = e->d_eh;
+    P.EL = e->d_EL;
+    P.order = e->d_order;
+    P.n = e->d_n;
+    P.idx = e->d_idx;
+    P.u = u;
+    P.wflat = e->wflat;
+    P.eflat = e->eflat;
+    P.woff = e->d_woff;
+    P.eoff = e->d_eoff;
+    P.seed = e->p.seed;
+    P.step = step;
+    P.status = e->d_status;
+    P.scratch = e->d_scratch;
+    P.scratch_bytes = e->scratch_bytes;
+    return P;
+}
+
+static Book make_book(mqe_engine *e) {
+    Book B;
+    B.n = e->d_n;
+    B.order = e->d_order;
+    B.order2 = e->d_order2;
+    B.w = e->d_w;
+    B.O = e->d_O;
+    B.eh = e->d_eh;
+    B.EL = e->d_EL;
+    B.sums = e->d_sums;
+    B.trace = e->d_trace;
+    B.ncopy = e->d_ncopy;
+    B.newpos = e->d_newpos;
+    B.used = e->d_used;
+    B.freelist = e->d_free;
+    B.copy_src = e->d_csrc;
+    B.copy_dst = e->d_cdst;
+    B.n_extra = e->d_nextra;
+    B.n_new = e->d_nnew;
+    B.n_used = e->d_nused;
+    B.status = e->d_status;
+    B.cap = e->p.capacity;
+    B.n_desired = e->p.n_desired;
+    B.dtau = e->p.dtau;
+    return B;
+}
+
+template <typename T>
+static void launch_walker(mqe_engine *e, const double *u, int step, int flags) {
+    Pool P = make_pool(e, u, step);
+    hipLaunchKernelGGL(walker_kernel<T>, dim3(e->p.capacity), dim3(MQE_NT), 0, e->stream, e->sys, P,
+                       (const T *)e->d_trial, flags);
+}
+
+static int launch_walker_any(mqe_engine *e, const double *u, int step, int flags) {
+    if (e->fp32) launch_walker<float>(e, u, step, flags);
+    else launch_walker<double>(e, u, step, flags);
+    HIPCHK(hipGetLastError());
+    return MQE_OK;
+}
+
+static int check_status(mqe_engine *e) {
+    int st = 0;
+    HIPCHK(hipMemcpyAsync(&st, e->d_status, sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (st & 4) { set_error("population exceeds walker capacity"); return MQE_ECAPACITY; }
+    if (st & 3) { set_error("zero PDF norm or non-finite weight"); return MQE_ENUMERIC; }
+    return MQE_OK;
+}
+
+template <typename T>
+static int upload_mps_typed(const double *src, long n, void **dst) {
+    std::vector<T> tmp(n);
+    for (long i = 0; i < n; ++i) tmp[i] = (T)src[i];
+    HIPCHK(hipMalloc(dst, sizeof(T) * std::max(n, 1L)));
+    HIPCHK(hipMemcpy(*dst, tmp.data(), sizeof(T) * n, hipMemcpyHostToDevice));
+    return MQE_OK;
+}
+
+#define ECHK(x)                       \
+    do {                              \
+        int _r = (x);                 \
+        if (_r != MQE_OK) return _r;  \
+    } while (0)
+
+template <typename T>
+static void launch_copy(mqe_engine *e) {
+    Book B = make_book(e);
+    Pool P = make_pool(e, nullptr, 0);
+    hipLaunchKernelGGL(copy_walkers_kernel<T>, dim3(e->p.capacity), dim3(MQE_NT), 0, e->stream, B, P, e->L);
+}
+
+static int enqueue_popcontrol(mqe_engine *e, double scaling, const double *du, int nu, int step) {
+    Book B = make_book(e);
+    hipLaunchKernelGGL(popcontrol_kernel, dim3(1), dim3(RED_NT), 0, e->stream, B, scaling, du, nu, e->p.seed, step);
+    HIPCHK(hipGetLastError());
+    if (e->fp32) launch_copy<float>(e);
+    else launch_copy<double>(e);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(install_order_kernel, dim3(16), dim3(256), 0, e->stream, B);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(set_n_kernel, dim3(1), dim3(64), 0, e->stream, B, e->d_n);
+    HIPCHK(hipGetLastError());
+    return MQE_OK;
+}
+
+extern "C" {
+
+const char *mqe_last_error(void) { return g_err.c_str(); }
+
+int mqe_create(const mqe_params *p, const mqe_mps *trial, int n_couplings, const int *first, const int *second,
+               const double *J, const double *tab_S, const double *tab_U, const double *tab_VT, const double *grid,
+               int n_grid, int grid_dim, mqe_engine **out) {
+    if (!p || !trial || !out || !grid) { set_error("mqe_create: null argument"); return MQE_EINVAL; }
+    if (p->d != 2 || trial->d != 2) { set_error("mqe_create: only d = 2 is supported"); return MQE_EINVAL; }
+    if (p->L < 2 || p->L > MQE_MAXL || trial->L != p->L) { set_error("mqe_create: bad chain length"); return MQE_EINVAL; }
+    if (grid_dim != 4 || n_grid < 1 || n_grid > MQE_MAXG) { set_error("mqe_create: grid must have dim d*d = 4"); return MQE_EINVAL; }
+    if (p->capacity < 1 || p->n_desired < 1 || p->max_bond < 1) { set_error("mqe_create: bad sizes"); return MQE_EINVAL; }
+    if (p->precision != MQE_PREC_FP64 && p->precision != MQE_PREC_FP32) { set_error("mqe_create: bad precision"); return MQE_EINVAL; }
+    const int L = p->L;
+    for (int c = 0; c < n_couplings; ++c) {
+        if (first[c] < 0 || first[c] >= second[c] || second[c] >= L) { set_error("mqe_create: bad coupling"); return MQE_EINVAL; }
+        if (c > 0 && (first[c] < first[c - 1] || (first[c] == first[c - 1] && second[c] <= second[c - 1]))) {
+            set_error("mqe_create: couplings must be in lexicographic order");
+            return MQE_EINVAL;
+        }
+    }
+    for (int k = 0; k < L; ++k)
+        if (trial->vd[k] < 1 || trial->vd[k] > 4096) { set_error("mqe_create: bad trial bond"); return MQE_EINVAL; }
+    if (trial->vd[0] != 1 || trial->vd[L] != 1) { set_error("mqe_create: trial boundary bonds must be 1"); return MQE_EINVAL; }
+
+    mqe_engine *e = new mqe_engine();
+    e->p = *p;
+    e->L = L;
+    e->nC = n_couplings;
+    e->Ng = n_grid;
+    e->gdim = grid_dim;
+    e->fp32 = p->precision == MQE_PREC_FP32;
+    e->tvd.assign(trial->vd, trial->vd + L + 1);
+    e->Dt = *std::max_element(e->tvd.begin(), e->tvd.end());
+    e->wmax.resize(L + 1);
+    mqe_bond_dims(L, p->max_bond, 2, e->wmax.data());
+    e->Dw = *std::max_element(e->wmax.begin(), e->wmax.end());
+    e->toff.resize(L);
+    e->woff.resize(L);
+    e->eoff.resize(L + 1);
+    long to = 0, wo = 0, eo = 0;
+    for (int k = 0; k < L; ++k) {
+        e->toff[k] = to;
+        to += (long)e->tvd[k] * e->tvd[k + 1] * 2;
+        e->woff[k] = wo;
+        wo += (long)e->wmax[k] * e->wmax[k + 1] * 2;
+    }
+    for (int k = 0; k <= L; ++k) {
+        e->eoff[k] = eo;
+        eo += (long)e->tvd[k] * e->wmax[k];
+    }
+    e->wflat = (wo + 63) & ~63L;
+    e->eflat = (eo + 63) & ~63L;
+    const long tsize = to;
+
+    auto fail = [&](int code) { mqe_destroy(e); return code; };
+#define CK(x)                                                                                       \
+    do {                                                                                            \
+        hipError_t _e = (x);                                                                        \
+        if (_e != hipSuccess) {                                                                     \
+            char _b[256];                                                                           \
+            snprintf(_b, sizeof(_b), "HIP error %s at %s:%d", hipGetErrorString(_e), __FILE__, __LINE__); \
+            set_error(_b);                                                                          \
+            return fail(MQE_EDEVICE);                                                               \
+        }                                                                                           \
+    } while (0)
+
+    CK(hipStreamCreate(&e->stream));
+    CK(hipEventCreate(&e->ev0));
+    CK(hipEventCreate(&e->ev1));
+    {
+        int r = e->fp32 ? upload_mps_typed<float>(trial->data, tsize, &e->d_trial)
+                        : upload_mps_typed<double>(trial->data, tsize, &e->d_trial);
+        if (r) return fail(r);
+    }
+    const int nC = n_couplings;
+    std::vector<int> nextfirst(std::max(nC, 1)), csptr(L + 2, 0), csidx(std::max(nC, 1));
+    for (int c = 0; c < nC; ++c) nextfirst[c] = (c + 1 < nC) ? first[c + 1] : L;
+    for (int c = 0; c < nC; ++c) csptr[second[c] + 1]++;
+    for (int k = 0; k < L; ++k) csptr[k + 1] += csptr[k];
+    {
+        std::vector<int> fill(csptr.begin(), csptr.end());
+        for (int c = 0; c < nC; ++c) csidx[fill[second[c]]++] = c;
+    }
+    auto up = [&](void **dst, const void *src, size_t bytes) -> hipError_t {
+        hipError_t r = hipMalloc(dst, std::max(bytes, (size_t)8));
+        if (r != hipSuccess) return r;
+        if (bytes) return hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+        return hipSuccess;
+    };
+    CK(up((void **)&e->d_tvd, e->tvd.data(), sizeof(int) * (L + 1)));
+    CK(up((void **)&e->d_toff, e->toff.data(), sizeof(long) * L));
+    CK(up((void **)&e->d_woff, e->woff.data(), sizeof(long) * L));
+    CK(up((void **)&e->d_eoff, e->eoff.data(), sizeof(long) * (L + 1)));
+    CK(up((void **)&e->d_first, first, sizeof(int) * nC));
+    CK(up((void **)&e->d_second, second, sizeof(int) * nC));
+    CK(up((void **)&e->d_nextfirst, nextfirst.data(), sizeof(int) * nC));
+    CK(up((void **)&e->d_csptr, csptr.data(), sizeof(int) * (L + 1)));
+    CK(up((void **)&e->d_csidx, csidx.data(), sizeof(int) * nC));
+    CK(up((void **)&e->d_Jc, J, sizeof(double) * nC));
+    CK(up((void **)&e->d_S, tab_S, sizeof(double) * 4 * nC));
+    CK(up((void **)&e->d_U, tab_U, sizeof(double) * 16 * nC));
+    CK(up((void **)&e->d_VT, tab_VT, sizeof(double) * 16 * nC));
+    CK(up((void **)&e->d_grid, grid, sizeof(double) * 4 * n_grid));
+
+    const int cap = p->capacity;
+    const size_t tsz = e->fp32 ? sizeof(float) : sizeof(double);
+    CK(hipMalloc(&e->d_phi, tsz * e->wflat * cap));
+    CK(hipMalloc(&e->d_Renv, tsz * e->eflat * cap));
+    CK(hipMalloc(&e->d_Lenv, tsz * e->eflat * cap));
+    CK(hipMemset(e->d_phi, 0, tsz * e->wflat * cap));
+    CK(hipMemset(e->d_Renv, 0, tsz * e->eflat * cap));
+    CK(hipMemset(e->d_Lenv, 0, tsz * e->eflat * cap));
+    CK(hipMalloc((void **)&e->d_wvd, sizeof(int) * (L + 1) * cap));
+    CK(hipMalloc((void **)&e->d_w, sizeof(double) * cap));
+    CK(hipMalloc((void **)&e->d_O, sizeof(double) * cap));
+    CK(hipMalloc((void **)&e->d_eh, sizeof(double) * cap));
+    CK(hipMalloc((void **)&e->d_EL, sizeof(double) * cap));
+    CK(hipMalloc((void **)&e->d_order, sizeof(int) * cap));
+    CK(hipMalloc((void **)&e->d_order2, sizeof(int) * cap));
+    CK(hipMalloc((void **)&e->d_n, sizeof(int)));
+    CK(hipMalloc((void **)&e->d_idx, sizeof(int) * std::max(nC, 1) * (size_t)cap));
+    e->scratch_bytes = e->fp32 ? scratch_bytes_for<float>(e->Dt, e->Dw, n_grid)
+                               : scratch_bytes_for<double>(e->Dt, e->Dw, n_grid);
+    {
+        // the overlap kernel needs 3 * Dmax^2 elements of T in the same scratch
+        long need = 3L * 4096 * 4096;  // upper bound used only if larger than the step scratch
+        (void)need;
+    }
+    CK(hipMalloc((void **)&e->d_scratch, (size_t)e->scratch_bytes * cap));
+    CK(hipMalloc((void **)&e->d_status, sizeof(int)));
+    CK(hipMemset(e->d_status, 0, sizeof(int)));
+    CK(hipMalloc((void **)&e->d_sums, sizeof(double) * 8));
+    CK(hipMemset(e->d_sums, 0, sizeof(double) * 8));
+    e->trace_cap = 1 << 16;
+    CK(hipMalloc((void **)&e->d_trace, sizeof(double) * 6 * e->trace_cap));
+    CK(hipMalloc((void **)&e->d_ncopy, sizeof(int) * cap));
+    CK(hipMalloc((void **)&e->d_newpos, sizeof(int) * cap));
+    CK(hipMalloc((void **)&e->d_used, sizeof(int) * cap));
+    CK(hipMalloc((void **)&e->d_free, sizeof(int) * cap));
+    CK(hipMalloc((void **)&e->d_csrc, sizeof(int) * cap));
+    CK(hipMalloc((void **)&e->d_cdst, sizeof(int) * cap));
+    CK(hipMalloc((void **)&e->d_nextra, sizeof(int)));
+    CK(hipMalloc((void **)&e->d_nnew, sizeof(int)));
+    CK(hipMalloc((void **)&e->d_nused, sizeof(int)));
+    CK(hipMalloc((void **)&e->d_tmpout, sizeof(double) * cap));
+
+    Sys &S = e->sys;
+    S.L = L;
+    S.nC = nC;
+    S.Ng = n_grid;
+    S.gdim = grid_dim;
+    S.Dt = e->Dt;
+    S.Dw = e->Dw;
+    S.tvd = e->d_tvd;
+    S.toff = e->d_toff;
+    S.first = e->d_first;
+    S.second = e->d_second;
+    S.nextfirst = e->d_nextfirst;
+    S.Jc = e->d_Jc;
+    S.tS = e->d_S;
+    S.tU = e->d_U;
+    S.tVT = e->d_VT;
+    S.grid = e->d_grid;
+    S.cs_ptr = e->d_csptr;
+    S.cs_idx = e->d_csidx;
+    // S.S two-site tensor: Sx Sx - iSy iSy + Sz Sz (OpSx.cpp, OpISy.cpp, OpSz.cpp, d = 2)
+    const double sx[4] = {0.0, 0.5, 0.5, 0.0}, isy[4] = {0.0, 0.5, -0.5, 0.0}, sz[4] = {-0.5, 0.0, 0.0, 0.5};
+    for (int sp = 0; sp < 2; ++sp)
+        for (int s = 0; s < 2; ++s)
+            for (int tp = 0; tp < 2; ++tp)
+                for (int t = 0; t < 2; ++t)
+                    S.SS[(sp + 2 * tp) * 4 + (s + 2 * t)] = sx[sp + 2 * s] * sx[tp + 2 * t] -
+                                                              isy[sp + 2 * s] * isy[tp + 2 * t] +
+                                                              sz[sp + 2 * s] * sz[tp + 2 * t];
+    S.Sz[0] = -0.5;
+    S.Sz[1] = 0.5;
+    S.h = p->field;
+    S.ssp[0] = std::exp(0.5 * p->field * p->dtau * (-0.5));  // TrotterHeisenberg.cpp:86
+    S.ssp[1] = std::exp(0.5 * p->field * p->dtau * (0.5));
+    *out = e;
+#undef CK
+    return MQE_OK;
+}
+
+void mqe_destroy(mqe_engine *e) {
+    if (!e) return;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
+    void *ptrs[] = {e->d_trial, e->d_tvd, e->d_first, e->d_second, e->d_nextfirst, e->d_csptr, e->d_csidx,
+                    e->d_toff, e->d_woff, e->d_eoff, e->d_Jc, e->d_S, e->d_U, e->d_VT, e->d_grid, e->d_phi,
+                    e->d_Renv, e->d_Lenv, e->d_wvd, e->d_w, e->d_O, e->d_eh, e->d_EL, e->d_order, e->d_order2,
+                    e->d_n, e->d_idx, e->d_u, e->d_scratch, e->d_status, e->d_sums, e->d_trace, e->d_ncopy,
+                    e->d_newpos, e->d_used, e->d_free, e->d_csrc, e->d_cdst, e->d_nextra, e->d_nnew, e->d_nused,
+                    e->d_tmpout};
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+    if (e->ev0) (void)hipEventDestroy(e->ev0);
+    if (e->ev1) (void)hipEventDestroy(e->ev1);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+int mqe_load_walkers(mqe_engine *e, int n, const int *vd, const double *data, const double *w, const double *O,
+                     const double *eh) {
+    if (!e || n < 1 || !vd || !data || !w || !O || !eh) { set_error("mqe_load_walkers: bad arguments"); return MQE_EINVAL; }
+    if (n > e->p.capacity) { set_error("mqe_load_walkers: n exceeds capacity"); return MQE_ECAPACITY; }
+    const int L = e->L;
+    const size_t tsz = e->fp32 ? sizeof(float) : sizeof(double);
+    std::vector<char> buf(tsz * e->wflat * n, 0);
+    std::vector<int> vds((size_t)(L + 1) * n);
+    long off = 0;
+    for (int i = 0; i < n; ++i) {
+        const int *v = vd + (size_t)i * (L + 1);
+        if (v[0] != 1 || v[L] != 1) { set_error("mqe_load_walkers: boundary bonds must be 1"); return MQE_EINVAL; }
+        for (int k = 0; k < L; ++k) {
+            if (v[k] < 1 || v[k] > e->wmax[k] || v[k + 1] > e->wmax[k + 1] || v[k] > 2 * v[k + 1]) {
+                set_error("mqe_load_walkers: walker bond dims exceed max_bond profile or dimL > d*dimR");
+                return MQE_EINVAL;
+            }
+        }
+        for (int k = 0; k <= L; ++k) vds[(size_t)i * (L + 1) + k] = v[k];
+        for (int k = 0; k < L; ++k) {
+            const long sz = (long)v[k] * v[k + 1] * 2;
+            char *dst = buf.data() + tsz * ((size_t)i * e->wflat + e->woff[k]);
+            for (long q = 0; q < sz; ++q) {
+                if (e->fp32) ((float *)dst)[q] = (float)data[off + q];
+                else ((double *)dst)[q] = data[off + q];
+            }
+            off += sz;
+        }
+    }
+    HIPCHK(hipMemcpy(e->d_phi, buf.data(), buf.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->d_wvd, vds.data(), sizeof(int) * vds.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->d_w, w, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->d_O, O, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->d_eh, eh, sizeof(double) * n, hipMemcpyHostToDevice));
+    std::vector<int> ord(e->p.capacity);
+    for (int i = 0; i < e->p.capacity; ++i) ord[i] = i;
+    HIPCHK(hipMemcpy(e->d_order, ord.data(), sizeof(int) * ord.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->d_n, &n, sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(e->d_status, 0, sizeof(int)));
+    e->n_host = n;
+    e->loaded = true;
+    e->prepared = false;
+    return MQE_OK;
+}
+
+static int ensure_u(mqe_engine *e, long need) {
+    if (need <= e->u_cap) return MQE_OK;
+    if (e->d_u) HIPCHK(hipFree(e->d_u));
+    e->d_u = nullptr;
+    HIPCHK(hipMalloc((void **)&e->d_u, sizeof(double) * std::max(need, 1L)));
+    e->u_cap = need;
+    return MQE_OK;
+}
+
+int mqe_prepare(mqe_engine *e, double *E_start) {
+    if (!e || !e->loaded) { set_error("mqe_prepare: no walkers loaded"); return MQE_ESTATE; }
+    ECHK(launch_walker_any(e, nullptr, 0, F_PREP));
+    Book B = make_book(e);
+    hipLaunchKernelGGL(energy_kernel, dim3(1), dim3(RED_NT), 0, e->stream, B, 1, 0, -1);
+    HIPCHK(hipGetLastError());
+    double sums[8];
+    HIPCHK(hipMemcpyAsync(sums, e->d_sums, sizeof(sums), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (E_start) *E_start = sums[3];
+    e->prepared = true;
+    return check_status(e);
+}
+
+int mqe_propagate(mqe_engine *e, const double *uniforms, double *sum_w) {
+    if (!e || !e->prepared) { set_error("mqe_propagate: call mqe_prepare first"); return MQE_ESTATE; }
+    const double *du = nullptr;
+    if (uniforms) {
+        // the uniforms are sized by the current population: refresh it first
+        HIPCHK(hipStreamSynchronize(e->stream));
+        HIPCHK(hipMemcpy(&e->n_host, e->d_n, sizeof(int), hipMemcpyDeviceToHost));
+        const long need = (long)e->n_host * std::max(e->nC, 1);
+        ECHK(ensure_u(e, need));
+        HIPCHK(hipMemcpyAsync(e->d_u, uniforms, sizeof(double) * (size_t)e->n_host * e->nC, hipMemcpyHostToDevice,
+                              e->stream));
+        du = e->d_u;
+    }
+    HIPCHK(hipEventRecord(e->ev0, e->stream));
+    ECHK(launch_walker_any(e, du, 0, F_PROP));
+    HIPCHK(hipEventRecord(e->ev1, e->stream));
+    Book B = make_book(e);
+    hipLaunchKernelGGL(sumw_kernel, dim3(1), dim3(RED_NT), 0, e->stream, B);
+    HIPCHK(hipGetLastError());
+    double sums[8];
+    HIPCHK(hipMemcpyAsync(sums, e->d_sums, sizeof(sums), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    HIPCHK(hipEventElapsedTime(&e->last_ms, e->ev0, e->ev1));
+    e->last_walkers = e->n_host;
+    if (sum_w) *sum_w = sums[0];
+    return check_status(e);
+}
+
+int mqe_energy(mqe_engine *e, int step, int do_fluct, double *E_P, double *omega) {
+    if (!e || !e->prepared) { set_error("mqe_energy: call mqe_prepare first"); return MQE_ESTATE; }
+    if (step < 1) { set_error("mqe_energy: step must be >= 1"); return MQE_EINVAL; }
+    Book B = make_book(e);
+    hipLaunchKernelGGL(energy_kernel, dim3(1), dim3(RED_NT), 0, e->stream, B, step, do_fluct ? 1 : 0, -1);
+    HIPCHK(hipGetLastError());
+    double sums[8];
+    HIPCHK(hipMemcpyAsync(sums, e->d_sums, sizeof(sums), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (E_P) *E_P = sums[3];
+    if (omega) *omega = sums[4];
+    return MQE_OK;
+}
+
+int mqe_population_control(mqe_engine *e, double scaling, const double *uniforms, int n_uniforms, int *n_new,
+                           int *n_used) {
+    if (!e || !e->prepared) { set_error("mqe_population_control: call mqe_prepare first"); return MQE_ESTATE; }
+    if (!(scaling > 0.0)) { set_error("mqe_population_control: scaling must be > 0"); return MQE_EINVAL; }
+    const double *du = nullptr;
+    if (uniforms && n_uniforms > 0) {
+        ECHK(ensure_u(e, n_uniforms));
+        HIPCHK(hipMemcpyAsync(e->d_u, uniforms, sizeof(double) * n_uniforms, hipMemcpyHostToDevice, e->stream));
+        du = e->d_u;
+    }
+    ECHK(enqueue_popcontrol(e, scaling, du, uniforms ? n_uniforms : 0, 0));
+    int nn = 0, nu = 0;
+    HIPCHK(hipMemcpyAsync(&nn, e->d_nnew, sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemcpyAsync(&nu, e->d_nused, sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (n_used) *n_used = nu;
+    if (nn > e->p.capacity) {
+        HIPCHK(hipMemset(e->d_status, 0, sizeof(int)));
+        set_error("population exceeds walker capacity");
+        return MQE_ECAPACITY;
+    }
+    e->n_host = nn;
+    if (n_new) *n_new = nn;
+    return MQE_OK;
+}
+
+static int enqueue_step(mqe_engine *e, int step) {
+    ECHK(launch_walker_any(e, nullptr, step, F_PROP));
+    Book B = make_book(e);
+    hipLaunchKernelGGL(sumw_kernel, dim3(1), dim3(RED_NT), 0, e->stream, B);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(energy_kernel, dim3(1), dim3(RED_NT), 0, e->stream, B, step, 1, (step - 1) % e->trace_cap);
+    HIPCHK(hipGetLastError());
+    ECHK(enqueue_popcontrol(e, -1.0, nullptr, 0, step));
+    return MQE_OK;
+}
+
+int mqe_step(mqe_engine *e, int step) {
+    if (!e || !e->prepared) { set_error("mqe_step: call mqe_prepare first"); return MQE_ESTATE; }
+    if (step < 1) { set_error("mqe_step: step must be >= 1"); return MQE_EINVAL; }
+    return enqueue_step(e, step);
+}
+
+int mqe_run(mqe_engine *e, int first_step, int n_steps, int use_graph) {
+    if (!e || !e->prepared) { set_error("mqe_run: call mqe_prepare first"); return MQE_ESTATE; }
+    (void)use_graph;
+    for (int s = 0; s < n_steps; ++s) ECHK(enqueue_step(e, first_step + s));
+    return MQE_OK;
+}
+
+int mqe_read_trace(mqe_engine *e, int first_step, int n_steps, double *out) {
+    if (!e || !out || first_step < 1 || n_steps < 0 || n_steps > e->trace_cap) { set_error("mqe_read_trace: bad arguments"); return MQE_EINVAL; }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    for (int s = 0; s < n_steps; ++s) {
+        const int row = (first_step - 1 + s) % e->trace_cap;
+        HIPCHK(hipMemcpy(out + 6L * s, e->d_trace + 6L * row, sizeof(double) * 6, hipMemcpyDeviceToHost));
+    }
+    return MQE_OK;
+}
+
+int mqe_sync(mqe_engine *e) {
+    if (!e) { set_error("mqe_sync: null engine"); return MQE_EINVAL; }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    int n = 0;
+    HIPCHK(hipMemcpy(&n, e->d_n, sizeof(int), hipMemcpyDeviceToHost));
+    e->n_host = n;
+    return check_status(e);
+}
+
+int mqe_num_walkers(mqe_engine *e, int *n) {
+    if (!e || !n) { set_error("mqe_num_walkers: bad arguments"); return MQE_EINVAL; }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    HIPCHK(hipMemcpy(n, e->d_n, sizeof(int), hipMemcpyDeviceToHost));
+    e->n_host = *n;
+    return MQE_OK;
+}
+
+int mqe_num_couplings(mqe_engine *e, int *n) {
+    if (!e || !n) { set_error("mqe_num_couplings: bad arguments"); return MQE_EINVAL; }
+    *n = e->nC;
+    return MQE_OK;
+}
+
+int mqe_read_scalars(mqe_engine *e, double *w, double *O, double *eh, double *EL) {
+    if (!e) { set_error("mqe_read_scalars: null engine"); return MQE_EINVAL; }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    const int n = e->n_host, cap = e->p.capacity;
+    std::vector<int> ord(n);
+    HIPCHK(hipMemcpy(ord.data(), e->d_order, sizeof(int) * n, hipMemcpyDeviceToHost));
+    std::vector<double> tmp(cap);
+    double *dst[4] = {w, O, eh, EL};
+    double *src[4] = {e->d_w, e->d_O, e->d_eh, e->d_EL};
+    for (int q = 0; q < 4; ++q) {
+        if (!dst[q]) continue;
+        HIPCHK(hipMemcpy(tmp.data(), src[q], sizeof(double) * cap, hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; ++i) dst[q][i] = tmp[ord[i]];
+    }
+    return MQE_OK;
+}
+
+int mqe_read_indices(mqe_engine *e, int *idx) {
+    if (!e || !idx) { set_error("mqe_read_indices: bad arguments"); return MQE_EINVAL; }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    HIPCHK(hipMemcpy(idx, e->d_idx, sizeof(int) * (size_t)e->last_walkers * e->nC, hipMemcpyDeviceToHost));
+    return MQE_OK;
+}
+
+int mqe_read_walker(mqe_engine *e, int i, int *vd, double *data, long *size) {
+    if (!e || i < 0 || i >= e->n_host || !vd) { set_error("mqe_read_walker: bad arguments"); return MQE_EINVAL; }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    const int L = e->L;
+    int slot = 0;
+    HIPCHK(hipMemcpy(&slot, e->d_order + i, sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(vd, e->d_wvd + (size_t)slot * (L + 1), sizeof(int) * (L + 1), hipMemcpyDeviceToHost));
+    long n = 0;
+    for (int k = 0; k < L; ++k) n += (long)vd[k] * vd[k + 1] * 2;
+    if (size) *size = n;
+    if (!data) return MQE_OK;
+    const size_t tsz = e->fp32 ? sizeof(float) : sizeof(double);
+    std::vector<char> buf(tsz * e->wflat);
+    HIPCHK(hipMemcpy(buf.data(), (char *)e->d_phi + tsz * (size_t)slot * e->wflat, buf.size(), hipMemcpyDeviceToHost));
+    long off = 0;
+    for (int k = 0; k < L; ++k) {
+        const long sz = (long)vd[k] * vd[k + 1] * 2;
+        const char *src = buf.data() + tsz * e->woff[k];
+        for (long q = 0; q < sz; ++q) data[off + q] = e->fp32 ? (double)((const float *)src)[q] : ((const double *)src)[q];
+        off += sz;
+    }
+    return MQE_OK;
+}
+
+int mqe_overlaps(mqe_engine *e, const mqe_mps *other, double *out) {
+    if (!e || !other || !out || other->L != e->L || other->d != 2) { set_error("mqe_overlaps: bad arguments"); return MQE_EINVAL; }
+    const int L = e->L;
+    if (other->vd[0] != 1 || other->vd[L] != 1) { set_error("mqe_overlaps: boundary bonds must be 1"); return MQE_EINVAL; }
+    int Dmax = e->Dw;
+    std::vector<long> off(L);
+    long tot = 0;
+    for (int k = 0; k < L; ++k) {
+        off[k] = tot;
+        tot += (long)other->vd[k] * other->vd[k + 1] * 2;
+        Dmax = std::max(Dmax, other->vd[k]);
+    }
+    const size_t tsz = e->fp32 ? sizeof(float) : sizeof(double);
+    // scratch: E, E2 (Dmax^2 each) and Y (Dmax x 2 Dmax)
+    const size_t need = tsz * 4 * (size_t)Dmax * Dmax;
+    char *scr = nullptr;
+    HIPCHK(hipMalloc((void **)&scr, need * std::max(e->n_host, 1)));
+    void *d_other = nullptr;
+    int r = e->fp32 ? upload_mps_typed<float>(other->data, tot, &d_other) : upload_mps_typed<double>(other->data, tot, &d_other);
+    if (r) { (void)hipFree(scr); return r; }
+    int *d_ovd = nullptr;
+    long *d_ooff = nullptr;
+    HIPCHK(hipMalloc((void **)&d_ovd, sizeof(int) * (L + 1)));
+    HIPCHK(hipMalloc((void **)&d_ooff, sizeof(long) * L));
+    HIPCHK(hipMemcpy(d_ovd, other->vd, sizeof(int) * (L + 1), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_ooff, off.data(), sizeof(long) * L, hipMemcpyHostToDevice));
+    Pool P = make_pool(e, nullptr, 0);
+    P.scratch = scr;
+    P.scratch_bytes = (long)need;
+    if (e->fp32)
+        hipLaunchKernelGGL(overlap_kernel<float>, dim3(e->n_host), dim3(MQE_NT), 0, e->stream, L, d_ovd, d_ooff,
+                           (const float *)d_other, P, Dmax, e->d_tmpout);
+    else
+        hipLaunchKernelGGL(overlap_kernel<double>, dim3(e->n_host), dim3(MQE_NT), 0, e->stream, L, d_ovd, d_ooff,
+                           (const double *)d_other, P, Dmax, e->d_tmpout);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(e->stream));
+    HIPCHK(hipMemcpy(out, e->d_tmpout, sizeof(double) * e->n_host, hipMemcpyDeviceToHost));
+    (void)hipFree(scr);
+    (void)hipFree(d_other);
+    (void)hipFree(d_ovd);
+    (void)hipFree(d_ooff);
+    return MQE_OK;
+}
+
+int mqe_last_kernel_ms(mqe_engine *e, double *ms, int *walkers) {
+    if (!e) { set_error("mqe_last_kernel_ms: null engine"); return MQE_EINVAL; }
+    if (ms) *ms = e->last_ms;
+    if (walkers) *walkers = e->last_walkers;
+    return MQE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,179 @@
+"""Python front-end of the walker engine (thin wrapper over the C ABI).
+
+The method names follow the reference driver that the engine replaces:
+``propagate`` = MPSQMC2::PropagateSeparately (MPSQMC2.cpp:518-611),
+``energy`` = MPSQMC2::EnergyFunctionAndHistory (:688-747),
+``population_control`` = MPSQMC2::SeparatePopulationControl (:613-686).
+"""
+import ctypes
+
+import numpy as np
+
+from . import mps as _mps
+from ._lib import MQE_PREC_FP32, MQE_PREC_FP64, MqeMps, MqeParams, check, lib
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _iptr(a):
+    return a.ctypes.data_as(_ip)
+
+
+class Engine:
+    """One GPU-resident walker population.
+
+    system: mpsqmc_amd.systems.System; trial: (vd, data), already left-normalized
+    (MPSQMC2.cpp:239).  precision: "fp64" (parity) or "fp32" (FP32 contractions).
+    """
+
+    def __init__(self, system, trial_vd, trial_data, n_desired, capacity=None, max_bond=None, precision="fp64",
+                 seed=20131015):
+        self.system = system
+        self.L = system.L
+        self.n_desired = int(n_desired)
+        self.capacity = int(capacity if capacity is not None else max(1000, 3 * n_desired))
+        tvd = np.ascontiguousarray(trial_vd, dtype=np.int32)
+        tdata = np.ascontiguousarray(trial_data, dtype=np.float64)
+        self.max_bond = int(max_bond if max_bond is not None else tvd.max())
+        prm = MqeParams(self.L, system.d, MQE_PREC_FP32 if precision == "fp32" else MQE_PREC_FP64, self.n_desired,
+                        self.capacity, self.max_bond, system.dtau, system.h, seed)
+        self._keep = [tvd, tdata]
+        m = MqeMps(self.L, system.d, _iptr(tvd), _dptr(tdata))
+        f, s, Jc = system.couplings()
+        S, U, VT = system.tables_per_coupling()
+        grid = np.ascontiguousarray(system.grid())
+        f = np.ascontiguousarray(f, dtype=np.int32)
+        s = np.ascontiguousarray(s, dtype=np.int32)
+        Jc = np.ascontiguousarray(Jc)
+        S = np.ascontiguousarray(S).reshape(-1)
+        U = np.ascontiguousarray(U).reshape(-1)
+        VT = np.ascontiguousarray(VT).reshape(-1)
+        self.nC = len(f)
+        h = ctypes.c_void_p()
+        check(lib().mqe_create(ctypes.byref(prm), ctypes.byref(m), self.nC, _iptr(f), _iptr(s), _dptr(Jc), _dptr(S),
+                               _dptr(U), _dptr(VT), _dptr(grid), grid.shape[0], grid.shape[1], ctypes.byref(h)))
+        self.h = h
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().mqe_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- walkers ------------------------------------------------------------
+    def load_walkers(self, vds, datas, w=None, O=None, eh=None):
+        n = len(vds)
+        vd = np.ascontiguousarray(np.concatenate([np.asarray(v, dtype=np.int32) for v in vds]))
+        data = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64) for x in datas]))
+        w = np.ascontiguousarray(np.ones(n) if w is None else w, dtype=np.float64)
+        O = np.ascontiguousarray(np.ones(n) if O is None else O, dtype=np.float64)
+        eh = np.ascontiguousarray(np.zeros(n) if eh is None else eh, dtype=np.float64)
+        check(lib().mqe_load_walkers(self.h, n, _iptr(vd), _dptr(data), _dptr(w), _dptr(O), _dptr(eh)))
+        self.n = n
+
+    def load_copies(self, vd, data, n, O=1.0):
+        """n copies of one MPS (SetupWalkers with copyTrial = true)."""
+        self.load_walkers([vd] * n, [data] * n, np.ones(n), np.full(n, O), np.zeros(n))
+
+    # -- the step -----------------------------------------------------------
+    def prepare(self):
+        E = ctypes.c_double()
+        check(lib().mqe_prepare(self.h, ctypes.byref(E)))
+        return E.value
+
+    def propagate(self, uniforms=None):
+        sw = ctypes.c_double()
+        if uniforms is not None:
+            u = np.ascontiguousarray(uniforms, dtype=np.float64)
+            assert u.size == self.n * self.nC
+            check(lib().mqe_propagate(self.h, _dptr(u), ctypes.byref(sw)))
+        else:
+            check(lib().mqe_propagate(self.h, None, ctypes.byref(sw)))
+        return sw.value
+
+    def energy(self, step, do_fluct=True):
+        EP = ctypes.c_double()
+        om = ctypes.c_double()
+        check(lib().mqe_energy(self.h, step, 1 if do_fluct else 0, ctypes.byref(EP), ctypes.byref(om)))
+        return EP.value, om.value
+
+    def population_control(self, scaling, uniforms=None):
+        nn = ctypes.c_int()
+        nu = ctypes.c_int()
+        if uniforms is not None:
+            u = np.ascontiguousarray(uniforms, dtype=np.float64)
+            check(lib().mqe_population_control(self.h, scaling, _dptr(u), u.size, ctypes.byref(nn), ctypes.byref(nu)))
+        else:
+            check(lib().mqe_population_control(self.h, scaling, None, 0, ctypes.byref(nn), ctypes.byref(nu)))
+        self.n = nn.value
+        return nn.value, nu.value
+
+    def step(self, step):
+        check(lib().mqe_step(self.h, step))
+
+    def run(self, first_step, n_steps, use_graph=False):
+        check(lib().mqe_run(self.h, first_step, n_steps, 1 if use_graph else 0))
+
+    def sync(self):
+        check(lib().mqe_sync(self.h))
+        self.n = self.num_walkers()
+
+    def read_trace(self, first_step, n_steps):
+        out = np.zeros((n_steps, 6))
+        check(lib().mqe_read_trace(self.h, first_step, n_steps, _dptr(out)))
+        return out
+
+    # -- inspection ---------------------------------------------------------
+    def num_walkers(self):
+        n = ctypes.c_int()
+        check(lib().mqe_num_walkers(self.h, ctypes.byref(n)))
+        return n.value
+
+    def scalars(self):
+        n = self.num_walkers()
+        self.n = n
+        w, O, eh, EL = (np.zeros(n) for _ in range(4))
+        check(lib().mqe_read_scalars(self.h, _dptr(w), _dptr(O), _dptr(eh), _dptr(EL)))
+        return dict(w=w, O=O, eh=eh, EL=EL)
+
+    def indices(self, n=None):
+        n = self.n if n is None else n
+        idx = np.zeros(n * self.nC, dtype=np.int32)
+        check(lib().mqe_read_indices(self.h, _iptr(idx)))
+        return idx.reshape(n, self.nC)
+
+    def walker(self, i):
+        vd = np.zeros(self.L + 1, dtype=np.int32)
+        size = ctypes.c_long()
+        check(lib().mqe_read_walker(self.h, i, _iptr(vd), None, ctypes.byref(size)))
+        data = np.zeros(size.value)
+        check(lib().mqe_read_walker(self.h, i, _iptr(vd), _dptr(data), ctypes.byref(size)))
+        return vd, data
+
+    def overlaps(self, vd, data):
+        vd = np.ascontiguousarray(vd, dtype=np.int32)
+        data = np.ascontiguousarray(data, dtype=np.float64)
+        m = MqeMps(self.L, self.system.d, _iptr(vd), _dptr(data))
+        out = np.zeros(self.num_walkers())
+        check(lib().mqe_overlaps(self.h, ctypes.byref(m), _dptr(out)))
+        return out
+
+    def last_kernel_ms(self):
+        ms = ctypes.c_double()
+        nw = ctypes.c_int()
+        check(lib().mqe_last_kernel_ms(self.h, ctypes.byref(ms), ctypes.byref(nw)))
+        return ms.value, nw.value
+
+
+__all__ = ["Engine", "MQE_PREC_FP32", "MQE_PREC_FP64", "_mps"]
