@@ -1,0 +1,152 @@
+#!/usr/bin/env python3
+"""Walker-steps/s of the MI355X walker engine on the L=32, D=32, 2048-walker
+Heisenberg chain (FP32 contractions; QR, sampling, weights, overlaps and the
+energy estimator in FP64), next to the reference algorithm on the host CPU.
+
+One step = one full Walk step of MPSQMC2 (MPSQMC2.cpp:465-511): propagation of
+every walker through all 31 couplings with normalisation, the local energy of
+every walker, the weight/energy reductions and population control.  Inputs are
+resident in HBM before the timed region; the device RNG supplies the
+auxiliary-field draws.
+
+Prints one JSON line (see the keys below).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# Algorithmic work per walker-step of the environment-sweep algorithm at
+# L=32, D=32 (d=2, 31 couplings): FP32 contraction flops and FP64 QR flops.
+F32_PER_WS = 61.5e6
+F64_PER_WS = 15.9e6
+PEAK_FP32 = 157.3e12   # MI355X dense FP32 matrix
+PEAK_FP64 = 78.6e12    # MI355X FP64
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--walkers", type=int, default=2048)
+    ap.add_argument("--L", type=int, default=32)
+    ap.add_argument("--D", type=int, default=32)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--cpu-walkers", type=int, default=128)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=20131015)
+    return ap.parse_args()
+
+
+def make_trial(L, D, seed):
+    from mpsqmc_amd import mps as M
+    vd, data = M.random_mps(L, D, 2, seed)
+    data, _ = M.left_normalize(vd, data)
+    return vd, data
+
+
+def cpu_baseline(system, vd, data, walkers, threads):
+    """The reference algorithm (16 + 1 full-chain overlaps per coupling, a full
+    QR left-normalisation after every operator, energy through H|trial>) in the
+    FP64 C restatement, OpenMP over walkers, on a bounded sample."""
+    import oracle
+    threads = max(1, min(threads, os.cpu_count() or 1))
+    pop = oracle.Population(system.L, 2, vd, data, system.J, system.h, system.dtau, system.tables_distinct(),
+                            system.grid().reshape(-1), 4, build_terms=True, threads=threads, energy_mode=2)
+    pop.init_walkers(walkers, int(vd.max()))
+    rng = np.random.default_rng(1)
+    u = rng.uniform(size=walkers * pop.nC)
+    t0 = time.perf_counter()
+    pop.propagate(u)
+    pop.energy(1, True)
+    dt = time.perf_counter() - t0
+    del pop
+    return dict(value=walkers / dt, unit="walker-steps/s", cores=threads, kind="port",
+                sample=f"{walkers} walkers x 1 step of the L={system.L}, D={int(vd.max())} chain "
+                       f"(reference algorithm, FP64, {threads} OpenMP threads), {dt:.1f} s")
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(walker sharding over GPUs is designed but not built)", file=sys.stderr)
+        sys.exit(2)
+    from mpsqmc_amd import systems
+    from mpsqmc_amd.engine import Engine
+
+    L, D, N = args.L, args.D, args.walkers
+    system = systems.chain(L, J=1.0, dtau=0.01)
+    vd, data = make_trial(L, D, args.seed)
+    eng = Engine(system, vd, data, n_desired=N, capacity=3 * N, max_bond=D, precision=args.precision,
+                 seed=args.seed)
+    eng.load_copies(vd, data, N)
+    eng.prepare()
+    log(f"bench: engine ready ({N} walkers, L={L}, D={D}, {args.precision}); warmup {args.warmup} steps")
+    eng.run(1, args.warmup)
+    eng.sync()
+    log("bench: warmup done; timing")
+    eng.set_timing(True)
+    first = args.warmup + 1
+    t0 = time.perf_counter()
+    eng.run(first, args.steps)
+    eng.sync()
+    t1 = time.perf_counter()
+    log(f"bench: {args.steps} steps in {t1 - t0:.3f} s")
+    ktot_ms, nlaunch = eng.kernel_time()
+    trace = eng.read_trace(first, args.steps)
+    walker_steps = float(trace[:, 1].sum())
+    wall = t1 - t0
+    value = walker_steps / wall
+    # roofline of the dominant kernel (the fused walker kernel)
+    avg_launch_s = ktot_ms / 1000.0 / max(nlaunch, 1)
+    walkers_per_launch = walker_steps / max(nlaunch, 1)
+    flops_per_launch = (F32_PER_WS + F64_PER_WS) * walkers_per_launch
+    achieved = flops_per_launch / avg_launch_s / 1e12
+    t_min = F32_PER_WS / PEAK_FP32 + F64_PER_WS / PEAK_FP64
+    peak = (F32_PER_WS + F64_PER_WS) / t_min / 1e12
+    out = {
+        "metric": "walker-steps/s (L=32, D=32, 2048 walkers)",
+        "value": value,
+        "unit": "walker-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1000.0,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if args.precision == "fp32" else "f64",
+        "data": "synthetic: random left-normalized trial MPS with the D=32 bond profile; walkers start as copies",
+        "config": {"workload": f"heisenberg-chain L={L} OBC J=1, trial D={D}, walkers D={D}, N={N}, dtau=0.01, "
+                               f"Marsaglia(4) grid, {args.precision} contractions",
+                   "walkers": N, "L": L, "D": D},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None,
+                     "kernel": "walker_kernel", "avg_launch_ms": avg_launch_s * 1000.0,
+                     "flops_per_walker_step": F32_PER_WS + F64_PER_WS},
+        "E_P_last": float(trace[-1, 2]),
+        "population_last": int(trace[-1, 1]),
+    }
+    eng.close()
+    if not args.no_cpu:
+        log(f"bench: CPU baseline on {args.cpu_walkers} walkers")
+        out["cpu_baseline"] = cpu_baseline(system, vd, data, args.cpu_walkers, args.cpu_threads)
+        out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -129,6 +129,12 @@ int mqe_overlaps(mqe_engine *e, const mqe_mps *other, double *out);
  * engine's stream), and the number of walkers it processed. */
 int mqe_last_kernel_ms(mqe_engine *e, double *ms, int *walkers);
 
+/* Per-launch timing of the walker kernel inside mqe_step / mqe_run: enable
+ * (resets the record), then read the summed device time (HIP events on the
+ * engine's stream) and the number of timed launches. */
+int mqe_set_timing(mqe_engine *e, int on);
+int mqe_kernel_time(mqe_engine *e, double *total_ms, int *launches);
+
 const char *mqe_last_error(void);
 
 /* ---- set-up helpers (host code; no GPU needed) ----------------------------- */

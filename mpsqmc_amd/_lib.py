@@ -70,6 +70,8 @@ SIGNATURES = {
     "mqe_read_walker": (c_int, [ctypes.c_void_p, c_int, P(c_int), P(c_double), P(c_long)]),
     "mqe_overlaps": (c_int, [ctypes.c_void_p, P(MqeMps), P(c_double)]),
     "mqe_last_kernel_ms": (c_int, [ctypes.c_void_p, P(c_double), P(c_int)]),
+    "mqe_set_timing": (c_int, [ctypes.c_void_p, c_int]),
+    "mqe_kernel_time": (c_int, [ctypes.c_void_p, P(c_double), P(c_int)]),
     "mqe_last_error": (ctypes.c_char_p, []),
     "mqe_bond_dims": (None, [c_int, c_int, c_int, P(c_int)]),
     "mqe_trotter_tables": (c_int, [c_int, c_double, c_double, P(c_double), P(c_double), P(c_double)]),

@@ -1005,6 +1005,10 @@ struct mqe_engine {
     float last_ms = 0.f;
     int last_walkers = 0;
     hipGraphExec_t graph_exec = nullptr;
+    // per-launch timing of the walker kernel inside mqe_step / mqe_run
+    bool timing = false;
+    std::vector<hipEvent_t> tev;
+    int tev_used = 0;
     Sys sys;
 };
 
@@ -1317,6 +1321,7 @@ void mqe_destroy(mqe_engine *e) {
                     e->d_tmpout};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
+    for (hipEvent_t ev : e->tev) (void)hipEventDestroy(ev);
     if (e->ev0) (void)hipEventDestroy(e->ev0);
     if (e->ev1) (void)hipEventDestroy(e->ev1);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -1460,7 +1465,21 @@ int mqe_population_control(mqe_engine *e, double scaling, const double *uniforms
 }
 
 static int enqueue_step(mqe_engine *e, int step) {
+    if (e->timing) {
+        if (e->tev_used + 2 > (int)e->tev.size()) {
+            for (int q = 0; q < 64; ++q) {
+                hipEvent_t ev;
+                HIPCHK(hipEventCreate(&ev));
+                e->tev.push_back(ev);
+            }
+        }
+        HIPCHK(hipEventRecord(e->tev[e->tev_used], e->stream));
+    }
     ECHK(launch_walker_any(e, nullptr, step, F_PROP));
+    if (e->timing) {
+        HIPCHK(hipEventRecord(e->tev[e->tev_used + 1], e->stream));
+        e->tev_used += 2;
+    }
     Book B = make_book(e);
     hipLaunchKernelGGL(sumw_kernel, dim3(1), dim3(RED_NT), 0, e->stream, B);
     HIPCHK(hipGetLastError());
@@ -1606,6 +1625,28 @@ int mqe_overlaps(mqe_engine *e, const mqe_mps *other, double *out) {
     (void)hipFree(d_other);
     (void)hipFree(d_ovd);
     (void)hipFree(d_ooff);
+    return MQE_OK;
+}
+
+int mqe_set_timing(mqe_engine *e, int on) {
+    if (!e) { set_error("mqe_set_timing: null engine"); return MQE_EINVAL; }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    e->timing = on != 0;
+    e->tev_used = 0;
+    return MQE_OK;
+}
+
+int mqe_kernel_time(mqe_engine *e, double *total_ms, int *launches) {
+    if (!e) { set_error("mqe_kernel_time: null engine"); return MQE_EINVAL; }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    double tot = 0.0;
+    for (int q = 0; q + 1 < e->tev_used; q += 2) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e->tev[q], e->tev[q + 1]));
+        tot += ms;
+    }
+    if (total_ms) *total_ms = tot;
+    if (launches) *launches = e->tev_used / 2;
     return MQE_OK;
 }
 

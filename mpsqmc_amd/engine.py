@@ -169,6 +169,15 @@ class Engine:
         check(lib().mqe_overlaps(self.h, ctypes.byref(m), _dptr(out)))
         return out
 
+    def set_timing(self, on=True):
+        check(lib().mqe_set_timing(self.h, 1 if on else 0))
+
+    def kernel_time(self):
+        ms = ctypes.c_double()
+        n = ctypes.c_int()
+        check(lib().mqe_kernel_time(self.h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
     def last_kernel_ms(self):
         ms = ctypes.c_double()
         nw = ctypes.c_int()

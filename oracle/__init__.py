@@ -43,6 +43,7 @@ def lib():
         L.orc_compress.restype = ctypes.c_int
         L.orc_apply_mpo.restype = ctypes.c_long
         L.orc_mpo_entry.restype = ctypes.c_double
+        L.orc_apply_mpo_min.restype = ctypes.c_long
         L.orc_pop_create.restype = ctypes.c_void_p
         L.orc_pop_create.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _dp, _dp, ctypes.c_double,
                                      ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _dp, _dp,
@@ -191,6 +192,17 @@ def apply_mpo(L, d, J, h, vd, data):
     n = lib().orc_apply_mpo(L, d, pJ, ctypes.c_double(h), pv, pd, vo.ctypes.data_as(_ip), None)
     out = np.zeros(n)
     lib().orc_apply_mpo(L, d, pJ, ctypes.c_double(h), pv, pd, vo.ctypes.data_as(_ip), out.ctypes.data_as(_dp))
+    return vo, out
+
+
+def apply_mpo_min(L, d, J, h, vd, data):
+    J, pJ = _d(J)
+    vd, pv = _i(vd)
+    data, pd = _d(data)
+    vo = np.zeros(L + 1, dtype=np.int32)
+    n = lib().orc_apply_mpo_min(L, d, pJ, ctypes.c_double(h), pv, pd, vo.ctypes.data_as(_ip), None)
+    out = np.zeros(n)
+    lib().orc_apply_mpo_min(L, d, pJ, ctypes.c_double(h), pv, pd, vo.ctypes.data_as(_ip), out.ctypes.data_as(_dp))
     return vo, out
 
 

@@ -695,6 +695,104 @@ static void apply_mpo(const ompo *M, const omps *psi, omps *out) {
     }
 }
 
+
+/* ------------------------------------------------------------------------- */
+/* H|trial> through a minimal-channel MPO: channels at cut k are "nothing yet",
+ * "complete", and S^a placed on a site i < k that still has a partner j >= k.
+ * The resulting MPS equals the reference's ApplyMPO + lossless CompressState
+ * (MPSQMC2.cpp:255-257) as a vector, with bond (2 + 3 n_open(k)) * D_T, the
+ * rank of H|trial> across the cut; it gives the reference's energy cost.     */
+/* ------------------------------------------------------------------------- */
+static int open_sites(const double *J, int L, int k, int *list) {
+    /* sites i < k with a nonzero coupling to some j >= k, in increasing order */
+    int n = 0;
+    for (int i = 0; i < k; i++) {
+        int has = 0;
+        for (int j = k; j < L && !has; j++) if (J[(j * (j - 1)) / 2 + i] != 0.0) has = 1;
+        if (has) list[n++] = i;
+    }
+    return n;
+}
+
+static void apply_mpo_min(int L, int d, const double *J, double h, const omps *psi, omps *out) {
+    int *nch = (int *)malloc(sizeof(int) * (L + 1));
+    int **lst = (int **)malloc(sizeof(int *) * (L + 1));
+    for (int k = 0; k <= L; k++) {
+        lst[k] = (int *)malloc(sizeof(int) * (L + 1));
+        int no = open_sites(J, L, k, lst[k]);
+        nch[k] = (k == 0 || k == L) ? 1 : 2 + 3 * no;
+        if (k > 0 && k < L) { /* keep the count in lst[k][L] */ lst[k][L] = no; }
+    }
+    /* channel numbering at an inner cut: 0 = nothing yet, 1 = complete, 2 + 3q + a = (lst[q], a) */
+    int *vd = (int *)malloc(sizeof(int) * (L + 1));
+    vd[0] = vd[L] = 1;
+    for (int k = 1; k < L; k++) vd[k] = nch[k] * psi->vd[k];
+    mps_alloc(out, L, d, vd);
+    free(vd);
+    const int ops[3] = {OP_SX, OP_ISY, OP_SZ};
+    const double sg[3] = {1.0, -1.0, 1.0};
+    double *O = (double *)malloc(sizeof(double) * d * d);
+    for (int k = 0; k < L; k++) {
+        int cl = nch[k], cr = nch[k + 1];
+        int dls = psi->vd[k], drs = psi->vd[k + 1];
+        int noL = (k > 0) ? lst[k][L] : 0, noR = (k + 1 < L) ? lst[k + 1][L] : 0;
+        /* map a left channel / right channel to (kind, site, a) */
+        for (int a = 0; a < cl; a++)
+            for (int b = 0; b < cr; b++) {
+                /* decode left */
+                int lk, li = -1, la = -1;   /* lk: 0 nothing, 1 complete, 2 open */
+                if (k == 0) lk = 0;
+                else if (a == 0) lk = 0;
+                else if (a == 1) lk = 1;
+                else { lk = 2; li = lst[k][(a - 2) / 3]; la = (a - 2) % 3; }
+                int rk, ri = -1, ra = -1;
+                if (k == L - 1) rk = 1;
+                else if (b == 0) rk = 0;
+                else if (b == 1) rk = 1;
+                else { rk = 2; ri = lst[k + 1][(b - 2) / 3]; ra = (b - 2) % 3; }
+                (void)noL; (void)noR;
+                double f = 0.0; int op = OP_ZERO;
+                if (lk == 0 && rk == 0) { f = 1.0; op = OP_I; }
+                else if (lk == 1 && rk == 1) { f = 1.0; op = OP_I; }
+                else if (lk == 0 && rk == 1) { if (h != 0.0) { f = -h; op = OP_SZ; } }
+                else if (lk == 0 && rk == 2) { if (ri == k) { f = 1.0; op = ops[ra]; } }
+                else if (lk == 2 && rk == 2) { if (li == ri && la == ra) { f = 1.0; op = OP_I; } }
+                else if (lk == 2 && rk == 1) {
+                    double v = J[(k * (k - 1)) / 2 + li];
+                    if (v != 0.0) { f = sg[la] * v; op = ops[la]; }
+                }
+                if (op == OP_ZERO) continue;
+                orc_operator(op, d, O);
+                for (int up = 0; up < d; up++) {
+                    double *tg = out->t[k] + (long)up * out->vd[k] * out->vd[k + 1];
+                    for (int dn = 0; dn < d; dn++) {
+                        double fb = f * O[up + d * dn];
+                        if (fb == 0.0) continue;
+                        const double *src = psi->t[k] + (long)dn * dls * drs;
+                        for (int r = 0; r < dls; r++)
+                            for (int c = 0; c < drs; c++)
+                                tg[(a * dls + r) + (dls * cl) * (b * drs + c)] += fb * src[r + dls * c];
+                    }
+                }
+            }
+    }
+    free(O);
+    for (int k = 0; k <= L; k++) free(lst[k]);
+    free(lst); free(nch);
+}
+
+ORC_API long orc_apply_mpo_min(int L, int d, const double *J, double h, const int *vd, const double *data,
+                               int *vd_out, double *out) {
+    omps P, H;
+    mps_from_flat(&P, L, d, vd, data);
+    apply_mpo_min(L, d, J, h, &P, &H);
+    memcpy(vd_out, H.vd, sizeof(int) * (L + 1));
+    long n = flat_size(L, d, H.vd);
+    if (out) mps_to_flat(&H, out);
+    mps_free(&P); mps_free(&H);
+    return n;
+}
+
 /* ------------------------------------------------------------------------- */
 /* CompressState (MPSstate.cpp:379-404) with TwoSiteObject Compose/Decompose  */
 /* (TwoSiteObject.cpp:71-184), Schmidt cutoff 1e-13 absolute (:16).           */
@@ -855,7 +953,10 @@ typedef struct {
     int *idx;               /* sampled grid point per (walker, coupling), last propagate */
     double *EL;             /* last local energies */
     int nthreads;
-    int emode;              /* 0: <H trial|phi> with uncompressed H|trial>; 1: per-coupling operator insertion */
+    int emode;              /* 0: <H trial|phi> with uncompressed H|trial>; 1: per-coupling operator insertion;
+                               2: <H trial|phi> with H|trial> from the minimal-channel MPO */
+    omps hmin;
+    double *Jfull;
 } orc_pop;
 
 ORC_API int orc_nthreads(void) {
@@ -941,6 +1042,9 @@ ORC_API orc_pop *orc_pop_create(int L, int d, const int *tvd, const double *tdat
     mpo_dims(L, M.dims);
     apply_mpo(&M, &p->trial, &p->htrial);
     free(M.dims);
+    p->Jfull = (double *)malloc(sizeof(double) * (L * (L - 1) / 2 > 0 ? L * (L - 1) / 2 : 1));
+    memcpy(p->Jfull, J, sizeof(double) * (L * (L - 1) / 2));
+    apply_mpo_min(L, d, J, h, &p->trial, &p->hmin);
     int nC = 0;
     for (int i = 0; i < L - 1; i++)
         for (int j = i + 1; j < L; j++) if (J[(j * (j - 1)) / 2 + i] != 0.0) nC++;
@@ -988,7 +1092,7 @@ ORC_API orc_pop *orc_pop_create(int L, int d, const int *tvd, const double *tdat
 
 ORC_API void orc_pop_destroy(orc_pop *p) {
     if (!p) return;
-    mps_free(&p->trial); mps_free(&p->htrial);
+    mps_free(&p->trial); mps_free(&p->htrial); mps_free(&p->hmin); free(p->Jfull);
     if (p->terms) {
         for (long i = 0; i < (long)p->nC * p->n2 * p->n2; i++) mps_free(&p->terms[i]);
         free(p->terms);
@@ -1192,7 +1296,8 @@ ORC_API double orc_pop_energy(orc_pop *p, int step, int dofluct, double *EP, dou
 #pragma omp parallel for schedule(static) num_threads(p->nthreads)
 #endif
     for (int i = 0; i < p->nw; i++)
-        e[i] = (p->emode == 0 ? inner(&p->htrial, &p->w[i]) : h_insert(p, &p->w[i])) / p->ov[i];
+        e[i] = (p->emode == 0 ? inner(&p->htrial, &p->w[i])
+                : (p->emode == 2 ? inner(&p->hmin, &p->w[i]) : h_insert(p, &p->w[i]))) / p->ov[i];
     for (int i = 0; i < p->nw; i++) {
         num += p->wt[i] * e[i];
         den += p->wt[i];

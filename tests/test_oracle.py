@@ -364,6 +364,19 @@ def test_energy_exact_trial_is_eigenvalue():
     assert np.allclose(trace[:, 2], E0, atol=1e-10)
 
 
+def test_minimal_mpo_matches_reference_mpo():
+    # same vector as ApplyMPO (HeisenbergMPO W tensors), smaller bonds
+    for sysm in (systems.chain(7, h=0.3), systems.snake_lattice(3, 3)):
+        L = sysm.L
+        vd, data = M.random_mps(L, 3, 2, 4)
+        v0, h0 = oracle.apply_mpo(L, 2, sysm.J, sysm.h, vd, data)
+        v1, h1 = oracle.apply_mpo_min(L, 2, sysm.J, sysm.h, vd, data)
+        assert np.allclose(M.to_dense(v0, h0), M.to_dense(v1, h1), atol=1e-12)
+        assert v1.max() <= v0.max()
+    v, _ = oracle.apply_mpo_min(32, 2, systems.chain(32).J, 0.0, *M.random_mps(32, 4, 2, 1))
+    assert v.max() == 5 * 4   # rank of H|psi> across a chain cut: 5 D
+
+
 def test_energy_modes_agree():
     L = 7
     sysm = systems.chain(L)
@@ -371,10 +384,9 @@ def test_energy_modes_agree():
     sysm.h = 0.3
     vd, data = random_trial(L, 4, 9)
     wv, wd = random_trial(L, 3, 10)
-    a = oracle_population(sysm, vd, data, build_terms=False, energy_mode=0)
-    b = oracle_population(sysm, vd, data, build_terms=False, energy_mode=1)
-    for p in (a, b):
+    pops = [oracle_population(sysm, vd, data, build_terms=False, energy_mode=m) for m in (0, 1, 2)]
+    for p in pops:
         p.set_walkers([wv, vd], [wd, data], np.ones(2), np.array([0.7, 1.0]), np.zeros(2))
-    ea = a.energy(1, False)[2]
-    eb = b.energy(1, False)[2]
+    ea, eb, ec = (p.energy(1, False)[2] for p in pops)
     assert np.allclose(ea, eb, rtol=1e-12)
+    assert np.allclose(ea, ec, rtol=1e-12)
