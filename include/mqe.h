@@ -107,6 +107,10 @@ int mqe_run(mqe_engine *e, int first_step, int n_steps, int use_graph);
 /* Copy the trace rows of steps first_step .. first_step+n_steps-1 (6 doubles each). */
 int mqe_read_trace(mqe_engine *e, int first_step, int n_steps, double *out);
 
+/* 1 when the engine runs the FP32 MFMA tile path (nearest-neighbour chain,
+ * all bonds <= 32, h = 0), 0 for the general path. */
+int mqe_kernel_path(mqe_engine *e);
+
 int mqe_sync(mqe_engine *e);
 int mqe_num_walkers(mqe_engine *e, int *n);
 int mqe_num_couplings(mqe_engine *e, int *n);

@@ -29,6 +29,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -79,6 +80,7 @@ struct Pool {
     int *status;
     char *scratch;
     long scratch_bytes;
+    int padded;            // 1: tile32 storage (32x32 zero-padded blocks per site and physical index)
 };
 
 // Scratch layout of one workgroup (sizes in elements).
@@ -716,19 +718,26 @@ __global__ __launch_bounds__(MQE_NT) void overlap_kernel(int L, const int *ovd, 
     set_one(E);
     for (int k = 0; k < L; ++k) {
         const int oa = ov[k], ob = ov[k + 1], wa = wv[k], wb = wv[k + 1];
-        const T *A = other + ooff[k], *Bt = phi + P.woff[k];
+        const T *A = other + ooff[k];
+        const T *B0 = P.padded ? phi + (size_t)(2 * k) * 1024 : phi + P.woff[k];
+        const T *B1 = P.padded ? phi + (size_t)(2 * k + 1) * 1024 : phi + P.woff[k] + (size_t)wa * wb;
+        const int ldb = P.padded ? 32 : wa;
         // Y = E A (wa x 2 ob) with E (wa x oa)
         gemm<false, false, T>(wa, 2 * ob, oa, E, wa, A, oa, Y, wa, false);
         __syncthreads();
         // E2 = sum_s B[s]^T Y[s]  (wb x ob)
-        gemm<true, false, T>(wb, ob, wa, Bt, wa, Y, wa, E2, wb, false);
+        gemm<true, false, T>(wb, ob, wa, B0, ldb, Y, wa, E2, wb, false);
         __syncthreads();
-        gemm<true, false, T>(wb, ob, wa, Bt + (size_t)wa * wb, wa, Y + (size_t)wa * ob, wa, E2, wb, true);
+        gemm<true, false, T>(wb, ob, wa, B1, ldb, Y + (size_t)wa * ob, wa, E2, wb, true);
         __syncthreads();
         T *t = E; E = E2; E2 = t;
     }
     if (threadIdx.x == 0) out[b] = (double)E[0];
 }
+
+}  // namespace mqe
+#include "mqe_tile32.h"
+namespace mqe {
 
 // ---------------------------------------------------------------------------
 // reductions, bookkeeping and population control (single workgroup, 1024 threads)
@@ -976,6 +985,8 @@ struct mqe_engine {
     mqe_params p;
     int L = 0, nC = 0, Ng = 0, gdim = 0, Dt = 0, Dw = 0;
     bool fp32 = false;
+    bool tile32 = false;   // FP32 MFMA path for NN chains with bonds <= 32, h = 0
+    float *d_t32N = nullptr, *d_t32T = nullptr;
     hipStream_t stream = nullptr;
     std::vector<int> tvd, wmax;
     std::vector<long> toff, woff, eoff;
@@ -1035,6 +1046,7 @@ static Pool make_pool(mqe_engine *e, const double *u, int step) {
     P.status = e->d_status;
     P.scratch = e->d_scratch;
     P.scratch_bytes = e->scratch_bytes;
+    P.padded = e->tile32 ? 1 : 0;
     return P;
 }
 
@@ -1073,7 +1085,11 @@ static void launch_walker(mqe_engine *e, const double *u, int step, int flags) {
 }
 
 static int launch_walker_any(mqe_engine *e, const double *u, int step, int flags) {
-    if (e->fp32) launch_walker<float>(e, u, step, flags);
+    if (e->tile32) {
+        Pool P = make_pool(e, u, step);
+        hipLaunchKernelGGL(t32::walker32_kernel, dim3(e->p.capacity), dim3(MQE_NT), t32::lds_bytes(), e->stream,
+                           e->sys, P, (const float *)e->d_t32N, (const float *)e->d_t32T, flags);
+    } else if (e->fp32) launch_walker<float>(e, u, step, flags);
     else launch_walker<double>(e, u, step, flags);
     HIPCHK(hipGetLastError());
     return MQE_OK;
@@ -1178,6 +1194,17 @@ int mqe_create(const mqe_params *p, const mqe_mps *trial, int n_couplings, const
     e->wflat = (wo + 63) & ~63L;
     e->eflat = (eo + 63) & ~63L;
     const long tsize = to;
+    {
+        bool nn = n_couplings == L - 1;
+        for (int c = 0; nn && c < n_couplings; ++c) nn = first[c] == c && second[c] == c + 1;
+        const char *no32 = getenv("MQE_NO_TILE32");
+        e->tile32 = e->fp32 && nn && e->Dt <= 32 && e->Dw <= 32 && p->field == 0.0 &&
+                    n_grid <= t32::MAXG_T32 && !(no32 && no32[0] == '1');
+        if (e->tile32) {
+            e->wflat = 2L * L * 1024;
+            e->eflat = (long)(L + 1) * 1024;
+        }
+    }
 
     auto fail = [&](int code) { mqe_destroy(e); return code; };
 #define CK(x)                                                                                       \
@@ -1198,6 +1225,26 @@ int mqe_create(const mqe_params *p, const mqe_mps *trial, int n_couplings, const
         int r = e->fp32 ? upload_mps_typed<float>(trial->data, tsize, &e->d_trial)
                         : upload_mps_typed<double>(trial->data, tsize, &e->d_trial);
         if (r) return fail(r);
+    }
+    if (e->tile32) {
+        // trial in zero-padded 32x32 blocks, plain and transposed
+        std::vector<float> tn((size_t)2 * L * 1024, 0.f), tt((size_t)2 * L * 1024, 0.f);
+        for (int k = 0; k < L; ++k) {
+            const int dl = e->tvd[k], dr = e->tvd[k + 1];
+            for (int s = 0; s < 2; ++s)
+                for (int c = 0; c < dr; ++c)
+                    for (int r = 0; r < dl; ++r) {
+                        const float v = (float)trial->data[e->toff[k] + r + (long)dl * (c + (long)dr * s)];
+                        tn[(size_t)(2 * k + s) * 1024 + r + 32 * c] = v;
+                        tt[(size_t)(2 * k + s) * 1024 + c + 32 * r] = v;
+                    }
+        }
+        CK(hipMalloc((void **)&e->d_t32N, sizeof(float) * tn.size()));
+        CK(hipMalloc((void **)&e->d_t32T, sizeof(float) * tt.size()));
+        CK(hipMemcpy(e->d_t32N, tn.data(), sizeof(float) * tn.size(), hipMemcpyHostToDevice));
+        CK(hipMemcpy(e->d_t32T, tt.data(), sizeof(float) * tt.size(), hipMemcpyHostToDevice));
+        CK(hipFuncSetAttribute((const void *)t32::walker32_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)t32::lds_bytes()));
     }
     const int nC = n_couplings;
     std::vector<int> nextfirst(std::max(nC, 1)), csptr(L + 2, 0), csidx(std::max(nC, 1));
@@ -1313,7 +1360,7 @@ void mqe_destroy(mqe_engine *e) {
     if (!e) return;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
-    void *ptrs[] = {e->d_trial, e->d_tvd, e->d_first, e->d_second, e->d_nextfirst, e->d_csptr, e->d_csidx,
+    void *ptrs[] = {e->d_t32N, e->d_t32T, e->d_trial, e->d_tvd, e->d_first, e->d_second, e->d_nextfirst, e->d_csptr, e->d_csidx,
                     e->d_toff, e->d_woff, e->d_eoff, e->d_Jc, e->d_S, e->d_U, e->d_VT, e->d_grid, e->d_phi,
                     e->d_Renv, e->d_Lenv, e->d_wvd, e->d_w, e->d_O, e->d_eh, e->d_EL, e->d_order, e->d_order2,
                     e->d_n, e->d_idx, e->d_u, e->d_scratch, e->d_status, e->d_sums, e->d_trace, e->d_ncopy,
@@ -1349,13 +1396,32 @@ int mqe_load_walkers(mqe_engine *e, int n, const int *vd, const double *data, co
         for (int k = 0; k <= L; ++k) vds[(size_t)i * (L + 1) + k] = v[k];
         for (int k = 0; k < L; ++k) {
             const long sz = (long)v[k] * v[k + 1] * 2;
-            char *dst = buf.data() + tsz * ((size_t)i * e->wflat + e->woff[k]);
-            for (long q = 0; q < sz; ++q) {
-                if (e->fp32) ((float *)dst)[q] = (float)data[off + q];
-                else ((double *)dst)[q] = data[off + q];
+            if (e->tile32) {
+                float *dst = (float *)buf.data() + (size_t)i * e->wflat + (size_t)2 * k * 1024;
+                const int dl = v[k], dr = v[k + 1];
+                for (int s2 = 0; s2 < 2; ++s2)
+                    for (int c = 0; c < dr; ++c)
+                        for (int r = 0; r < dl; ++r)
+                            dst[(size_t)s2 * 1024 + r + 32 * c] = (float)data[off + r + (long)dl * (c + (long)dr * s2)];
+            } else {
+                char *dst = buf.data() + tsz * ((size_t)i * e->wflat + e->woff[k]);
+                for (long q = 0; q < sz; ++q) {
+                    if (e->fp32) ((float *)dst)[q] = (float)data[off + q];
+                    else ((double *)dst)[q] = data[off + q];
+                }
             }
             off += sz;
         }
+    }
+    if (e->tile32) {
+        // L_0 = R_L = 1 in every slot (boundary environments read from global memory)
+        std::vector<float> env((size_t)e->eflat * e->p.capacity, 0.f);
+        for (int sl = 0; sl < e->p.capacity; ++sl) {
+            env[(size_t)sl * e->eflat] = 1.f;
+            env[(size_t)sl * e->eflat + (size_t)L * 1024] = 1.f;
+        }
+        HIPCHK(hipMemcpy(e->d_Lenv, env.data(), sizeof(float) * env.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(e->d_Renv, env.data(), sizeof(float) * env.size(), hipMemcpyHostToDevice));
     }
     HIPCHK(hipMemcpy(e->d_phi, buf.data(), buf.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(e->d_wvd, vds.data(), sizeof(int) * vds.size(), hipMemcpyHostToDevice));
@@ -1512,6 +1578,8 @@ int mqe_read_trace(mqe_engine *e, int first_step, int n_steps, double *out) {
     return MQE_OK;
 }
 
+int mqe_kernel_path(mqe_engine *e) { return (e && e->tile32) ? 1 : 0; }
+
 int mqe_sync(mqe_engine *e) {
     if (!e) { set_error("mqe_sync: null engine"); return MQE_EINVAL; }
     HIPCHK(hipStreamSynchronize(e->stream));
@@ -1576,8 +1644,17 @@ int mqe_read_walker(mqe_engine *e, int i, int *vd, double *data, long *size) {
     long off = 0;
     for (int k = 0; k < L; ++k) {
         const long sz = (long)vd[k] * vd[k + 1] * 2;
-        const char *src = buf.data() + tsz * e->woff[k];
-        for (long q = 0; q < sz; ++q) data[off + q] = e->fp32 ? (double)((const float *)src)[q] : ((const double *)src)[q];
+        if (e->tile32) {
+            const float *src = (const float *)buf.data() + (size_t)2 * k * 1024;
+            const int dl = vd[k], dr = vd[k + 1];
+            for (int s2 = 0; s2 < 2; ++s2)
+                for (int c = 0; c < dr; ++c)
+                    for (int r = 0; r < dl; ++r)
+                        data[off + r + (long)dl * (c + (long)dr * s2)] = (double)src[(size_t)s2 * 1024 + r + 32 * c];
+        } else {
+            const char *src = buf.data() + tsz * e->woff[k];
+            for (long q = 0; q < sz; ++q) data[off + q] = e->fp32 ? (double)((const float *)src)[q] : ((const double *)src)[q];
+        }
         off += sz;
     }
     return MQE_OK;

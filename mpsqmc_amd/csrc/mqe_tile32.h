@@ -1,0 +1,564 @@
+// mqe_tile32.h -- FP32 MFMA walker kernel for nearest-neighbour chains with
+// all trial and walker bond dimensions <= 32 (the L=32, D=32 production case).
+//
+// Storage (global memory, zero-padded so every contraction is a full 32x32x32
+// MFMA tile):
+//   walker site k, physical s:  phi + (2k+s)*1024, element (r, c) at r + 32c
+//   environment k (L_k / R_k):  env + k*1024,      element (trial a, walker b) at a + 32b
+//   trial:  T32N[(2k+s)*1024] = T_k[s] (padded), T32T = T_k[s]^T (padded)
+// LDS holds the working set of one walker: 32x32 blocks with leading dim 33
+// (conflict-free for both row- and column-wise MFMA operand reads).
+//
+// One 256-thread workgroup (four wave64s) per walker.  Every 32x32x32 product
+// is one wave issuing 16 v_mfma_f32_32x32x2f32; the four waves work on
+// independent tiles between workgroup barriers.  QR (LQ) of the energy sweep is
+// FP64 Householder in LDS with one barrier per column; sampling, weights,
+// overlaps and energies are FP64.
+#ifndef MQE_TILE32_H
+#define MQE_TILE32_H
+
+namespace mqe {
+namespace t32 {
+
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+constexpr int LD = 33;           // LDS leading dimension
+constexpr int BLK = 32 * LD;     // floats per LDS block
+constexpr int GB = 1024;         // floats per global block
+
+// LDS map (in LDS blocks).  The center/next walker tensors alternate between
+// blocks 0-1 and 2-3.  During the LQ the FP64 workspace overlays blocks 7-15.
+enum : int {
+    B_BUF0 = 0,  // 2 blocks
+    B_BUF1 = 2,  // 2 blocks
+    B_L = 4,     // left environment
+    B_R = 5,     // right environment
+    B_LM = 6,    // Lmat of the LQ
+    B_N = 7,     // left norm environment
+    B_Y = 8,     // 2 blocks
+    B_W = 10,    // 2 blocks (also PDF/CDF scratch during sampling)
+    B_V = 12,    // 4 blocks (X after the pair tensor)
+    NBLK = 16
+};
+constexpr int QLD = 65;                         // FP64 QR workspace leading dim
+constexpr int Q_TAU = 64 * QLD;                 // tau after the 64 x 65 matrix
+constexpr int MAXG_T32 = (2 * BLK * 4) / 16;    // grid points that fit PDF + CDF in B_W
+
+constexpr size_t lds_bytes() { return sizeof(float) * (size_t)NBLK * BLK; }
+
+__device__ __forceinline__ v16f zero16() {
+    v16f z;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) z[q] = 0.f;
+    return z;
+}
+
+// acc += op(A)(32 x 32) op(B)(32 x 32); TA: A stored transposed; TB: B stored transposed.
+template <bool TA, bool TB>
+__device__ __forceinline__ void mma(v16f &acc, const float *A, int lda, const float *B, int ldb) {
+    const int l = threadIdx.x & 63, r = l & 31, h = l >> 5;
+#pragma unroll
+    for (int k0 = 0; k0 < 32; k0 += 2) {
+        const int k = k0 + h;
+        const float a = TA ? A[k + lda * r] : A[r + lda * k];
+        const float b = TB ? B[r + ldb * k] : B[k + ldb * r];
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+    }
+}
+
+// element (i, j) held by this lane in accumulator slot q
+__device__ __forceinline__ int row_of(int q) { return 8 * (q >> 2) + 4 * ((threadIdx.x & 63) >> 5) + (q & 3); }
+__device__ __forceinline__ int col_of() { return threadIdx.x & 31; }
+
+__device__ __forceinline__ void store(const v16f &acc, float *C, int ldc) {
+    const int j = col_of();
+#pragma unroll
+    for (int q = 0; q < 16; ++q) C[row_of(q) + ldc * j] = acc[q];
+}
+
+__device__ __forceinline__ double dot_tile(const v16f &acc, const float *C, int ldc) {
+    const int j = col_of();
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) s += (double)acc[q] * (double)C[row_of(q) + ldc * j];
+    return wave_sum(s);
+}
+
+// global padded block -> LDS block (whole workgroup)
+__device__ __forceinline__ void g2l(const float *g, float *s) {
+    for (int e = threadIdx.x; e < GB; e += MQE_NT) s[(e & 31) + LD * (e >> 5)] = g[e];
+}
+__device__ __forceinline__ void l2g(const float *s, float *g) {
+    for (int e = threadIdx.x; e < GB; e += MQE_NT) g[e] = s[(e & 31) + LD * (e >> 5)];
+}
+__device__ __forceinline__ void set_unit(float *s) {  // 1 at (0,0), 0 elsewhere
+    for (int e = threadIdx.x; e < BLK; e += MQE_NT) s[e] = (e == 0) ? 1.f : 0.f;
+}
+
+struct W32 {
+    const Sys *S;
+    const float *TN, *TT;   // trial padded blocks
+    float *phi, *Lg, *Rg;   // this walker's global storage
+    float *sm;              // LDS floats
+    const int *wv;          // walker bond dims (LDS)
+    double *red;            // LDS reduction scratch
+    __device__ float *blk(int b) const { return sm + (size_t)b * BLK; }
+    __device__ const float *tn(int k, int s) const { return TN + (size_t)(2 * k + s) * GB; }
+    __device__ const float *tt(int k, int s) const { return TT + (size_t)(2 * k + s) * GB; }
+    __device__ float *ph(int k, int s) const { return phi + (size_t)(2 * k + s) * GB; }
+};
+
+// ---------------------------------------------------------------------------
+// FP64 Householder LQ of the center tensor (two LDS blocks, w_k x 2 w_{k+1}):
+// phi = Lmat Q with Q right-orthonormal.  Q overwrites the center blocks and
+// Lmat (lower triangular, w_k x w_k) goes to B_LM.  The transposed matrix
+// A = phi^T (m = 2 w_{k+1} rows, n = w_k columns) is factorised; row index
+// r = col + w_{k+1} s.  32 groups of 8 lanes, group g owns column g.
+// ---------------------------------------------------------------------------
+__device__ void lq32(const W32 &C, int wk, int wn, float *center) {
+    const int m = 2 * wn, n = wk;
+    double *A = (double *)C.blk(B_N), *tau = A + Q_TAU;
+    const int tid = threadIdx.x, g = tid >> 3, ln = tid & 7;
+    for (int e = tid; e < 64 * 32; e += MQE_NT) {
+        const int r = e & 63, c = e >> 6;
+        double v = 0.0;
+        if (r < m && c < n) {
+            const int s = r >= wn ? 1 : 0, col = r - s * wn;
+            v = (double)center[s * BLK + c + LD * col];
+        }
+        A[r + QLD * c] = v;
+    }
+    __syncthreads();
+    // reflector of column 0 by group 0
+    auto make_reflector = [&](int j) {
+        // group j: column j, rows j..m-1
+        double *cj = A + QLD * j;
+        double ss = 0.0;
+        for (int i = j + 1 + ln; i < m; i += 8) ss += cj[i] * cj[i];
+        ss += __shfl_xor(ss, 1, 64);
+        ss += __shfl_xor(ss, 2, 64);
+        ss += __shfl_xor(ss, 4, 64);
+        const double alpha = cj[j];
+        double tj = 0.0;
+        if (ss != 0.0) {
+            const double beta = -copysign(sqrt(alpha * alpha + ss), alpha);
+            tj = (beta - alpha) / beta;
+            const double sc = 1.0 / (alpha - beta);
+            for (int i = j + 1 + ln; i < m; i += 8) cj[i] *= sc;
+            // all 8 lanes read alpha above before lane 0 writes beta: lanes of a
+            // group are in one wave and execute in lockstep
+            if (ln == 0) cj[j] = beta;
+        }
+        if (ln == 0) tau[j] = tj;
+    };
+    if (g == 0) make_reflector(0);
+    __syncthreads();
+    for (int j = 0; j < n; ++j) {
+        const double tj = tau[j];
+        const double *vj = A + QLD * j;
+        if (g > j && g < n && tj != 0.0) {
+            double *cg = A + QLD * g;
+            double s = 0.0;
+            for (int i = j + 1 + ln; i < m; i += 8) s += vj[i] * cg[i];
+            s += __shfl_xor(s, 1, 64);
+            s += __shfl_xor(s, 2, 64);
+            s += __shfl_xor(s, 4, 64);
+            s = tj * (s + cg[j]);
+            for (int i = j + ln; i < m; i += 8) cg[i] -= s * (i == j ? 1.0 : vj[i]);
+        }
+        if (g == j + 1 && g < n) make_reflector(j + 1);
+        __syncthreads();
+    }
+    // Lmat = R^T (lower), from the upper triangle of A
+    float *LM = C.blk(B_LM);
+    for (int e = tid; e < 32 * 32; e += MQE_NT) {
+        const int a = e & 31, b = e >> 5;
+        LM[a + LD * b] = (a < n && b <= a) ? (float)A[b + QLD * a] : 0.f;
+    }
+    // Q column c = H_0 ... H_c e_c, group c; result written transposed into the center
+    if (g < n) {
+        const int c = g;
+        double q[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int i = ln + 8 * t;
+            q[t] = (i == c) ? 1.0 : 0.0;
+        }
+        for (int k = c; k >= 0; --k) {
+            const double tk = tau[k];
+            if (tk == 0.0) continue;
+            const double *vk = A + QLD * k;
+            double s = 0.0;
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int i = ln + 8 * t;
+                if (i >= k && i < m) s += (i == k ? 1.0 : vk[i]) * q[t];
+            }
+            s += __shfl_xor(s, 1, 64);
+            s += __shfl_xor(s, 2, 64);
+            s += __shfl_xor(s, 4, 64);
+            s *= tk;
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int i = ln + 8 * t;
+                if (i >= k && i < m) q[t] -= s * (i == k ? 1.0 : vk[i]);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int i = ln + 8 * t;
+            if (i < m) {
+                const int s = i >= wn ? 1 : 0, col = i - s * wn;
+                center[s * BLK + c + LD * col] = (float)q[t];
+            }
+        }
+    }
+    __syncthreads();
+}
+
+
+// ---------------------------------------------------------------------------
+// the walker kernel (flags: F_PREP or F_PROP), h = 0, couplings (i, i+1)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void scale_blocks(float *b, int nb, double f) {
+    for (int e = threadIdx.x; e < nb * BLK; e += MQE_NT) b[e] = (float)((double)b[e] * f);
+}
+
+// phi[s] <- sum_s0 O[s + 2 s0] phi[s0] on a 2-block LDS tensor
+__device__ __forceinline__ void apply2(float *b, const double *O) {
+    const float o00 = (float)O[0], o10 = (float)O[1], o01 = (float)O[2], o11 = (float)O[3];
+    for (int e = threadIdx.x; e < BLK; e += MQE_NT) {
+        const float a0 = b[e], a1 = b[e + BLK];
+        b[e] = o00 * a0 + o01 * a1;
+        b[e + BLK] = o10 * a0 + o11 * a1;
+    }
+}
+
+// <trial|phi> with the walker's site-0 tensor in LDS block pair c0 and R_1 in B_R;
+// returns {overlap, squared norm of site 0}
+__device__ void site0_32(const W32 &C, const float *c0, double *ov, double *n2) {
+    const float *R1 = C.blk(B_R);
+    double a = 0.0, b = 0.0;
+    for (int e = threadIdx.x; e < 2 * 32 * 32; e += MQE_NT) {
+        const int s = e >> 10, al = (e >> 5) & 31, be = e & 31;
+        const double t = (double)C.tn(0, s)[0 + 32 * al];
+        const double p = (double)c0[s * BLK + 0 + LD * be];
+        a += t * p * (double)R1[al + LD * be];
+        if (al == 0) b += p * p;
+    }
+    double v[2] = {a, b};
+    block_sum_n<2>(v, C.red);
+    *ov = v[0];
+    *n2 = v[1];
+}
+
+// Energy sweep (right -> left) from the center phi_{L-1} (LDS pair pc); L_k in
+// global memory.  Returns <trial|H|phi>; *Ou = <trial|phi>; normalises site 0.
+__device__ double energy_sweep32(const W32 &C, int pc, double *On, double *Ou) {
+    const Sys &S = *C.S;
+    const int L = S.L;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __shared__ double Mq[16];
+    set_unit(C.blk(B_R));
+    double etot = 0.0;
+    int pj = pc == B_BUF0 ? B_BUF1 : B_BUF0;
+    for (int k = L - 1; k >= 1; --k) {
+        float *cen = C.blk(pc), *nxt = C.blk(pj);
+        g2l(C.ph(k - 1, 0), nxt);
+        g2l(C.ph(k - 1, 1), nxt + BLK);
+        g2l(C.Lg + (size_t)(k - 1) * GB, C.blk(B_L));
+        __syncthreads();
+        lq32(C, C.wv[k], C.wv[k + 1], cen);  // cen <- Q_k, B_LM <- Lmat
+        // stage 1: push (waves 0,1, in place), W[t] = R_{k+1} Q_k[t]^T (waves 2,3)
+        {
+            v16f acc = zero16();
+            if (w < 2) {
+                mma<false, false>(acc, nxt + w * BLK, LD, C.blk(B_LM), LD);
+                store(acc, nxt + w * BLK, LD);
+            } else {
+                const int t = w - 2;
+                mma<false, true>(acc, C.blk(B_R), LD, cen + t * BLK, LD);
+                store(acc, C.blk(B_W) + t * BLK, LD);
+            }
+        }
+        __syncthreads();
+        // stage 2: V(t',t) = T_k[t'] W[t] (4 waves); Y[s] = L_{k-1} phi'_{k-1}[s] (waves 0,1)
+        {
+            v16f acc = zero16();
+            mma<false, false>(acc, C.tn(k, w >> 1), 32, C.blk(B_W) + (w & 1) * BLK, LD);
+            store(acc, C.blk(B_V) + w * BLK, LD);
+            if (w < 2) {
+                v16f y = zero16();
+                mma<false, false>(y, C.blk(B_L), LD, nxt + w * BLK, LD);
+                store(y, C.blk(B_Y) + w * BLK, LD);
+            }
+        }
+        // write Q_k (final for this step) to global while the products finish
+        l2g(cen, C.ph(k, 0));
+        l2g(cen + BLK, C.ph(k, 1));
+        __syncthreads();
+        // stage 3: R_k = V(0,0) + V(1,1); bond energy of (k-1, k) from G = T_{k-1}^T Y and V
+        {
+            const float *V = C.blk(B_V);
+            float *R = C.blk(B_R);
+            for (int e = threadIdx.x; e < BLK; e += MQE_NT) R[e] = V[e] + V[3 * BLK + e];
+            v16f g = zero16();
+            mma<false, false>(g, C.tt(k - 1, w >> 1), 32, C.blk(B_Y) + (w & 1) * BLK, LD);
+            const int sp = w >> 1, ss = w & 1;
+            for (int q = 0; q < 4; ++q) {
+                const double d = dot_tile(g, V + q * BLK, LD);
+                if (lane == 0) Mq[(sp + 2 * (q >> 1)) * 4 + (ss + 2 * (q & 1))] = d;
+            }
+        }
+        __syncthreads();
+        l2g(C.blk(B_R), C.Rg + (size_t)k * GB);
+        double e = 0.0;
+        for (int q = 0; q < 16; ++q) e += S.SS[q] * Mq[q];
+        etot += S.Jc[k - 1] * e;
+        int t = pc; pc = pj; pj = t;
+        __syncthreads();
+    }
+    double ov, n2;
+    float *c0 = C.blk(pc);
+    site0_32(C, c0, &ov, &n2);
+    const double nrm = sqrt(n2);
+    double O = ov / nrm, f = 1.0 / nrm;
+    if (O < 0.0) { O = -O; f = -f; }
+    scale_blocks(c0, 2, f);
+    __syncthreads();
+    l2g(c0, C.ph(0, 0));
+    l2g(c0 + BLK, C.ph(0, 1));
+    *On = O;
+    *Ou = ov;
+    return etot;
+}
+
+__global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, const float *TN, const float *TT,
+                                                              int flags) {
+    extern __shared__ float lds32[];
+    const int b = blockIdx.x;
+    const int n = *P.n;
+    if (b >= n) return;
+    const int slot = P.order[b];
+    const int L = S.L;
+    __shared__ int wv[MQE_MAXL + 1];
+    __shared__ double red[MQE_NT + 16 * MQE_WAVES];
+    __shared__ double M[16], comb[16], Ux[4], Vx[4], scal[4];
+    for (int k = threadIdx.x; k <= L; k += MQE_NT) wv[k] = P.wvd[(size_t)slot * (L + 1) + k];
+    W32 C;
+    C.S = &S;
+    C.TN = TN;
+    C.TT = TT;
+    C.phi = (float *)P.phi + (size_t)slot * P.wflat;
+    C.Lg = (float *)P.Lenv + (size_t)slot * P.eflat;
+    C.Rg = (float *)P.Renv + (size_t)slot * P.eflat;
+    C.sm = lds32;
+    C.wv = wv;
+    C.red = red;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double wt = P.w[slot], O = P.O[slot];
+
+    if (flags & F_PREP) {
+        // left environments of the loaded state
+        set_unit(C.blk(B_L));
+        __syncthreads();
+        float *cen = C.blk(B_BUF0);
+        for (int k = 0; k < L - 1; ++k) {
+            g2l(C.ph(k, 0), cen);
+            g2l(C.ph(k, 1), cen + BLK);
+            __syncthreads();
+            if (w < 2) {
+                v16f y = zero16();
+                mma<false, false>(y, C.blk(B_L), LD, cen + w * BLK, LD);
+                store(y, C.blk(B_Y) + w * BLK, LD);
+            }
+            __syncthreads();
+            if (w == 0) {
+                v16f a = zero16();
+                mma<false, false>(a, C.tt(k, 0), 32, C.blk(B_Y), LD);
+                mma<false, false>(a, C.tt(k, 1), 32, C.blk(B_Y) + BLK, LD);
+                store(a, C.blk(B_L), LD);
+            }
+            __syncthreads();
+            l2g(C.blk(B_L), C.Lg + (size_t)(k + 1) * GB);
+            __syncthreads();
+        }
+        g2l(C.ph(L - 1, 0), cen);
+        g2l(C.ph(L - 1, 1), cen + BLK);
+        __syncthreads();
+        double On, Ou;
+        const double e = energy_sweep32(C, B_BUF0, &On, &Ou);
+        if (threadIdx.x == 0) {
+            P.EL[slot] = e / O;
+            P.O[slot] = On;
+        }
+        return;
+    }
+
+    // ---------------- propagation (MPSQMC2.cpp:532-600), h = 0 ----------------
+    int pc = B_BUF0, pj = B_BUF1;
+    g2l(C.ph(0, 0), C.blk(pc));
+    g2l(C.ph(0, 1), C.blk(pc) + BLK);
+    set_unit(C.blk(B_L));
+    set_unit(C.blk(B_N));
+    __syncthreads();
+    for (int c = 0; c < S.nC; ++c) {
+        const int i = c, j = c + 1;
+        float *cen = C.blk(pc), *nxt = C.blk(pj);
+        g2l(C.ph(j, 0), nxt);
+        g2l(C.ph(j, 1), nxt + BLK);
+        g2l(C.Rg + (size_t)(j + 1) * GB, C.blk(B_R));
+        __syncthreads();
+        // stage A: Y[s] = L_i phi_i[s] (waves 0,1); W[t] = R_{j+1} phi_j[t]^T (waves 2,3)
+        {
+            v16f acc = zero16();
+            if (w < 2) {
+                mma<false, false>(acc, C.blk(B_L), LD, cen + w * BLK, LD);
+                store(acc, C.blk(B_Y) + w * BLK, LD);
+            } else {
+                mma<false, true>(acc, C.blk(B_R), LD, nxt + (w - 2) * BLK, LD);
+                store(acc, C.blk(B_W) + (w - 2) * BLK, LD);
+            }
+        }
+        __syncthreads();
+        // stage B: G(s',s) = T_i[s']^T Y[s] kept in registers; V(t',t) = T_j[t'] W[t] to LDS
+        const int sp = w >> 1, ss = w & 1;
+        v16f g = zero16();
+        mma<false, false>(g, C.tt(i, sp), 32, C.blk(B_Y) + ss * BLK, LD);
+        {
+            v16f v = zero16();
+            mma<false, false>(v, C.tn(j, w >> 1), 32, C.blk(B_W) + (w & 1) * BLK, LD);
+            store(v, C.blk(B_V) + w * BLK, LD);
+        }
+        __syncthreads();
+        // stage C: the 16 two-site mixed matrix elements
+        for (int q = 0; q < 4; ++q) {
+            const double d = dot_tile(g, C.blk(B_V) + q * BLK, LD);
+            if (lane == 0) M[(sp + 2 * (q >> 1)) * 4 + (ss + 2 * (q & 1))] = d;
+        }
+        __syncthreads();
+        // stage D: combos, PDF, CDF sample (MPSQMC2.cpp:540-588)
+        const double *tS = S.tS + 4 * c, *tU = S.tU + 16 * c, *tVT = S.tVT + 16 * c;
+        if (threadIdx.x < 16) {
+            const int a = threadIdx.x & 3, bb = threadIdx.x >> 2;
+            double sacc = 0.0;
+            for (int xsp = 0; xsp < 2; ++xsp)
+                for (int xss = 0; xss < 2; ++xss) {
+                    const double ua = tU[xsp + 2 * (xss + 2 * a)];
+                    if (ua == 0.0) continue;
+                    for (int tp = 0; tp < 2; ++tp)
+                        for (int t = 0; t < 2; ++t)
+                            sacc += ua * tVT[bb + 4 * (tp + 2 * t)] * M[(xsp + 2 * tp) * 4 + (xss + 2 * t)];
+                }
+            comb[threadIdx.x] = sqrt(tS[a] * tS[bb]) * sacc;
+        }
+        __syncthreads();
+        double *pdf = (double *)C.blk(B_W), *cum = pdf + MAXG_T32;
+        const int Ng = S.Ng;
+        const double pref = 4.0 / (O * Ng);
+        double part = 0.0;
+        for (int gq = threadIdx.x; gq < Ng; gq += MQE_NT) {
+            const double *x = S.grid + 4 * gq;
+            double f = 0.0;
+            for (int l = 0; l < 4; ++l)
+                for (int r = 0; r < 4; ++r) f += comb[l + 4 * r] * x[l] * x[r];
+            double v = f > 0.0 ? f : 0.0;
+            v *= pref;
+            pdf[gq] = v;
+            part += v;
+        }
+        const double pdfnorm = block_sum(part, red);
+        const double inv = 1.0 / pdfnorm;
+        for (int gq = threadIdx.x; gq < Ng; gq += MQE_NT) pdf[gq] *= inv;
+        __syncthreads();
+        block_inclusive_scan(Ng, pdf, cum, red);
+        const double u = P.u ? P.u[(size_t)b * S.nC + c]
+                             : philox_uniform(P.seed, (uint32_t)b, (uint32_t)c, (uint32_t)P.step, 0u);
+        double cnt = 0.0;
+        for (int gq = threadIdx.x; gq < Ng - 1; gq += MQE_NT) cnt += (cum[gq] <= u) ? 1.0 : 0.0;
+        cnt = block_sum(cnt, red);
+        const int gs = (int)cnt;
+        wt *= pdfnorm;
+        if (threadIdx.x == 0) {
+            if (P.idx) P.idx[(size_t)b * S.nC + c] = gs;
+            if (!(pdfnorm > 0.0) || !isfinite(pdfnorm)) atomicOr(P.status, 1);
+            const double *x = S.grid + 4 * gs;
+            double f = 0.0;
+            for (int l = 0; l < 4; ++l)
+                for (int r = 0; r < 4; ++r) f += comb[l + 4 * r] * x[l] * x[r];
+            scal[0] = f;
+            for (int q = 0; q < 4; ++q) { Ux[q] = 0.0; Vx[q] = 0.0; }
+            for (int kk = 0; kk < 4; ++kk) {
+                const double pf = sqrt(tS[kk] * 4.0) * x[kk];
+                if (pf == 0.0) continue;
+                for (int up = 0; up < 2; ++up)
+                    for (int dn = 0; dn < 2; ++dn) {
+                        Ux[up + 2 * dn] += pf * tU[up + 2 * (dn + 2 * kk)];
+                        Vx[up + 2 * dn] += pf * tVT[kk + 4 * (up + 2 * dn)];
+                    }
+            }
+        }
+        __syncthreads();
+        const double fstar = scal[0];
+        // stage E: apply the sampled operators; each wave scales its G tile for L_{i+1}
+        apply2(cen, Ux);
+        apply2(nxt, Vx);
+        {
+            const float ucoef = (float)Ux[sp + 2 * ss];
+            v16f gs2 = g * ucoef;
+            float *dst = (w < 2) ? C.blk(B_Y) + w * BLK : C.blk(B_W) + (w - 2) * BLK;
+            store(gs2, dst, LD);
+        }
+        __syncthreads();
+        // stage F: L_{i+1} = sum of the four scaled tiles; X[s] = N_i phi'_i[s]; phi'_i is final
+        {
+            float *Ln = C.blk(B_L);
+            const float *y0 = C.blk(B_Y), *w0 = C.blk(B_W);
+            for (int e = threadIdx.x; e < BLK; e += MQE_NT) Ln[e] = y0[e] + y0[BLK + e] + w0[e] + w0[BLK + e];
+            l2g(cen, C.ph(i, 0));
+            l2g(cen + BLK, C.ph(i, 1));
+            if (w < 2) {
+                v16f x = zero16();
+                mma<false, false>(x, C.blk(B_N), LD, cen + w * BLK, LD);
+                store(x, C.blk(B_V) + w * BLK, LD);
+            }
+        }
+        __syncthreads();
+        // stage G: N_{i+1} = sum_s phi'_i[s]^T X[s] (wave 0); L_{i+1} to global
+        if (w == 0) {
+            v16f a = zero16();
+            mma<true, false>(a, cen, LD, C.blk(B_V), LD);
+            mma<true, false>(a, cen + BLK, LD, C.blk(B_V) + BLK, LD);
+            store(a, C.blk(B_N), LD);
+        }
+        l2g(C.blk(B_L), C.Lg + (size_t)(i + 1) * GB);
+        __syncthreads();
+        // stage H: squared norm = sum_t <phi'_j[t], N_{i+1} phi'_j[t]> (right part right-orthonormal)
+        if (w < 2) {
+            v16f z = zero16();
+            mma<false, false>(z, C.blk(B_N), LD, nxt + w * BLK, LD);
+            const double d = dot_tile(z, nxt + w * BLK, LD);
+            if (lane == 0) scal[1 + w] = d;
+        }
+        __syncthreads();
+        const double nrm = sqrt(scal[1] + scal[2]);
+        scale_blocks(nxt, 2, 1.0 / nrm);
+        O = 4.0 * fstar / nrm;
+        int t = pc; pc = pj; pj = t;
+        __syncthreads();
+    }
+    // ---------------- local energy (Walker.cpp:72-77) ----------------
+    double On, Ou;
+    const double e = energy_sweep32(C, pc, &On, &Ou);
+    if (threadIdx.x == 0) {
+        P.w[slot] = wt;
+        P.O[slot] = On;
+        P.EL[slot] = e / Ou;
+        if (!isfinite(wt)) atomicOr(P.status, 2);
+    }
+}
+
+}  // namespace t32
+}  // namespace mqe
+
+#endif

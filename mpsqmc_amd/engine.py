@@ -125,6 +125,11 @@ class Engine:
     def run(self, first_step, n_steps, use_graph=False):
         check(lib().mqe_run(self.h, first_step, n_steps, 1 if use_graph else 0))
 
+    @property
+    def path(self):
+        """'tile32' (FP32 MFMA path) or 'general'."""
+        return "tile32" if lib().mqe_kernel_path(self.h) == 1 else "general"
+
     def sync(self):
         check(lib().mqe_sync(self.h))
         self.n = self.num_walkers()
