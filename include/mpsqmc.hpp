@@ -1,0 +1,187 @@
+// mpsqmc.hpp -- C++ host classes with the reference's driver interface
+// (include/MPSQMC2.h, Walker.h, MPSstate.h, TrotterHeisenberg.h,
+// GridGenerator.h, HeisenbergMPO.h, Random.h), implemented over the engine's
+// C ABI (mqe.h).  MPSQMC2::Walk keeps the reference's call order
+// (MPSQMC2.cpp:443-513): propagate all walkers, scaling and E_T, projected
+// energy and fluctuation metric, trace line, population control -- with the
+// walker population resident on the GPU.
+#ifndef MPSQMC_HPP
+#define MPSQMC_HPP
+
+#include <ostream>
+#include <string>
+#include <vector>
+
+#include "mqe.h"
+
+namespace mpsqmc_amd {
+
+// Random.h: one seeded mt19937 stream, u = x * 2^-32 (the reference seeds
+// from time(0), Random.cpp:31; a fixed seed makes runs reproducible).
+class Random {
+  public:
+    explicit Random(unsigned int seed = 20131015u);
+    ~Random();
+    Random(const Random &) = delete;
+    Random &operator=(const Random &) = delete;
+    double rand();
+    void fill(long n, double *out);
+    void peek(long n, double *out);
+    void discard(long k);
+
+  private:
+    mqe_rng *rng_;
+};
+
+// GridGenerator.h:13-34
+class GridGenerator {
+  public:
+    explicit GridGenerator(int dim);
+    int gDim() const { return dim_; }
+    int gNPoints() const { return n_; }
+    double gCoOfPoint(int point, int index) const { return grid_[index + dim_ * point]; }
+    void FillMarsaglia(int pointsPerCo);
+    const double *data() const { return grid_.data(); }
+
+  private:
+    int dim_;
+    int n_ = 0;
+    std::vector<double> grid_;
+};
+
+// HeisenbergMPO.h:25-52 (couplings and field; H = sum J_ij S_i.S_j - h sum S^z_i)
+class HeisenbergMPO {
+  public:
+    HeisenbergMPO(int length, int phys_d, bool useLadder = false);
+    void sCoupling(int i, int j, double value);
+    double gCoupling(int i, int j) const;
+    void sField(double value) { field_ = value; }
+    double gField() const { return field_; }
+    int gLength() const { return L_; }
+    int gPhys_d() const { return d_; }
+
+  private:
+    int L_, d_;
+    double field_ = 0.0;
+    std::vector<double> J_;  // J[(j*(j-1))/2 + i], i < j
+};
+
+// TrotterHeisenberg.h:16-40
+class TrotterHeisenberg {
+  public:
+    TrotterHeisenberg(const HeisenbergMPO *mpo, double dtau);
+    bool gIsMagneticField() const { return field_ != 0.0; }
+    double gField() const { return field_; }
+    double gCoupling(int i, int j) const { return mpo_->gCoupling(i, j); }
+    double gTwoSitePropSVD_Sing(double J, int k) const;
+    double gTwoSitePropSVD_Left(double J, int k, int i, int j) const;
+    double gTwoSitePropSVD_Right(double J, int k, int i, int j) const;
+    double gSingleSiteProp(int i, int j) const;
+    // flattened per-coupling tables for a coupling list
+    void tables(const std::vector<double> &Jc, std::vector<double> &S, std::vector<double> &U,
+                std::vector<double> &VT) const;
+
+  private:
+    int find(double J) const;
+    const HeisenbergMPO *mpo_;
+    double dtau_, field_;
+    std::vector<double> dJ_, S_, U_, VT_;
+};
+
+// MPSstate.h:19-102 (the host-side MPS; its FP64 methods are used for set-up)
+class MPSstate {
+  public:
+    MPSstate(int length, int Dtrunc, int phys_d, Random *RN);
+    MPSstate(const char *filename, Random *RN);
+    explicit MPSstate(const MPSstate *toCopy);
+    MPSstate(int length, int phys_d, const std::vector<int> &vd, const std::vector<double> &data);
+    int gLength() const { return L_; }
+    int gPhys_d() const { return d_; }
+    int gDtrunc() const { return Dtrunc_; }
+    int gDimAtBound(int bound) const { return (bound < 0 || bound > L_) ? 0 : vd_[bound]; }
+    double LeftNormalize();
+    double InnerProduct(const MPSstate *other) const;
+    void ChangePhase();
+    void CompressState(int truncD);
+    void CompressState() { CompressState(Dtrunc_); }
+    const std::vector<int> &bonds() const { return vd_; }
+    const std::vector<double> &data() const { return data_; }
+    std::vector<double> &data() { return data_; }
+    friend std::ostream &operator<<(std::ostream &os, const MPSstate &m);
+
+  private:
+    long offset(int site) const;
+    int L_, Dtrunc_, d_;
+    std::vector<int> vd_;
+    std::vector<double> data_;
+};
+
+// Walker.h:13-52 (a host copy of one GPU walker)
+class Walker {
+  public:
+    Walker(const MPSstate *state, double overlap, double weight, double energyHistory)
+        : state_(state), overlap_(overlap), weight_(weight), eh_(energyHistory) {}
+    double gWeight() const { return weight_; }
+    double gOverlap() const { return overlap_; }
+    double gEnergyHistory() const { return eh_; }
+    const MPSstate *gState() const { return &state_; }
+
+  private:
+    MPSstate state_;
+    double overlap_, weight_, eh_;
+};
+
+struct StepRecord {
+    int step;
+    int nwalkers;
+    double projectedEnergy, targetEnergy, fluctMetric, sumWeights;
+};
+
+// MPSQMC2.h:21,27 with the trial given (the DMRG-running constructor is not provided)
+class MPSQMC2 {
+  public:
+    enum class Rng { Host, Device };
+    MPSQMC2(HeisenbergMPO *theMPO, GridGenerator *theGrid, Random *RN, MPSstate *Psi0_in, int DW, int Nwalkers,
+            double dtau, int precision = MQE_PREC_FP64, Rng rng = Rng::Host);
+    ~MPSQMC2();
+    MPSQMC2(const MPSQMC2 &) = delete;
+    MPSQMC2 &operator=(const MPSQMC2 &) = delete;
+
+    // Let the walkers propagate for steps steps (MPSQMC2.cpp:443-513)
+    void Walk(int steps);
+
+    // trace file: the reference writes output/Heisenberg1D/ener_L%dDT%dDW%d.txt (MPSQMC2.cpp:452,752)
+    void setTraceFile(const std::string &path) { trace_path_ = path; }
+    void setVerbose(bool v) { verbose_ = v; }
+    const std::vector<StepRecord> &history() const { return history_; }
+    double energyAtStart() const { return E_start_; }
+    int nWalkers() const;
+    Walker walker(int i) const;
+    mqe_engine *engine() { return eng_; }
+
+  private:
+    double PropagateSeparately();
+    void SeparatePopulationControl(double scaling);
+    double EnergyFunctionAndHistory(int step, double *projectedEnergy, bool doFluctuationMetric = true);
+    void write(int step, int nwalkers, double projectedEnergy, double targetEnergy, double fluctMetric);
+
+    HeisenbergMPO *theMPO_;
+    GridGenerator *theGrid_;
+    Random *RN_;
+    TrotterHeisenberg *theTrotter_;
+    MPSstate *Psi0_;
+    int DT_, DW_, totalNDesiredWalkers_, totalNCurrentWalkers_, nCouplings_;
+    double dtau_;
+    Rng rng_;
+    mqe_engine *eng_ = nullptr;
+    std::string trace_path_;
+    bool verbose_ = false;
+    bool started_ = false;
+    int step_offset_ = 0;
+    double E_start_ = 0.0;
+    std::vector<StepRecord> history_;
+};
+
+}  // namespace mpsqmc_amd
+
+#endif

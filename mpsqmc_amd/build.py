@@ -12,7 +12,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmqe.so")
 
-SOURCES = ["mqe_engine.hip", "mqe_host.cpp"]
+SOURCES = ["mqe_engine.hip", "mqe_host.cpp", "mpsqmc.cpp"]
+WALK = os.path.join(HERE, "mpsqmc_walk")
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -35,6 +36,13 @@ def build(verbose=False):
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
     os.replace(LIB + ".tmp", LIB)
+    # the command-line walk driver over the C++ host classes
+    cmd2 = [hipcc, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), os.path.join(CSRC, "mpsqmc_main.cpp"),
+            "-L", HERE, "-lmqe", "-Wl,-rpath,$ORIGIN", "-o", WALK + ".tmp"]
+    if verbose:
+        print(" ".join(cmd2), flush=True)
+    subprocess.check_call(cmd2)
+    os.replace(WALK + ".tmp", WALK)
     return LIB
 
 

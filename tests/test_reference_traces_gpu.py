@@ -1,0 +1,64 @@
+"""Statistical agreement with the reference's own shipped runs
+(output/Heisenberg1D/ener_L50DT4DW{2,4}.txt: L = 50 open chain, trial
+input/Heisenberg1D/L50D4.mps, 1000 walkers, dtau = 0.01).  The reference runs
+are time-seeded, so agreement is within the statistical error of E_P,
+sqrt(Omega / N) per step (Omega from the same file)."""
+import os
+
+import numpy as np
+import pytest
+
+from mpsqmc_amd import mps as M
+from mpsqmc_amd import systems
+from mpsqmc_amd.engine import Engine
+
+pytestmark = [pytest.mark.gpu, pytest.mark.usefixtures("require_gpu")]
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _reference_trace(name):
+    rows = []
+    with open(os.path.join(GOLDEN, name)) as f:
+        for ln in f:
+            if ln.startswith("#") or not ln.strip():
+                continue
+            rows.append([float(x) for x in ln.split()])
+    return np.array(rows)  # step, N, E_P, E_T, Omega
+
+
+@pytest.mark.parametrize("DW,fname,precision", [(4, "ener_L50DT4DW4.txt", "fp64"), (2, "ener_L50DT4DW2.txt", "fp32")])
+def test_l50_reference_runs(DW, fname, precision):
+    L, Dt, d, vd, data = M.read_mps(os.path.join(GOLDEN, "L50D4.mps"))
+    data, _ = M.left_normalize(vd, data)
+    s = systems.chain(L)
+    ref = _reference_trace(fname)
+    nsteps = ref.shape[0]
+    N = 1000
+    if DW < Dt:
+        wv, wd = M.compress(vd, data, DW)
+        wd, _ = M.left_normalize(wv, wd)
+        import oracle
+        O = oracle.inner(L, 2, vd, data, wv, wd)
+        if O < 0:
+            wd = wd.copy()
+            n0 = int(wv[0] * wv[1] * 2)
+            wd[:n0] *= -1
+            O = -O
+    else:
+        wv, wd, O = vd, data, 1.0
+    eng = Engine(s, vd, data, n_desired=N, capacity=3 * N, max_bond=DW, precision=precision, seed=2013)
+    eng.load_copies(wv, wd, N, O=O)
+    E0 = eng.prepare()
+    if DW == Dt:
+        assert abs(E0 - (-21.863425804367605)) < 1e-8   # E_var of L50D4.mps
+    eng.run(1, nsteps)
+    tr = eng.read_trace(1, nsteps)
+    eng.close()
+    # per-step statistical error of the mixed estimator from the reference's Omega
+    err = np.sqrt(np.maximum(ref[:, 4], 1e-12) / N)
+    z = np.abs(tr[:, 2] - ref[:, 2]) / err
+    assert z[0] < 6, (tr[0, 2], ref[0, 2], err[0])
+    assert np.median(z) < 6, z
+    # the population stays controlled around N
+    assert np.all(np.abs(tr[:, 1] - N) < 0.1 * N)
