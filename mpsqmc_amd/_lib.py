@@ -64,6 +64,7 @@ SIGNATURES = {
     "mqe_read_trace": (c_int, [ctypes.c_void_p, c_int, c_int, P(c_double)]),
     "mqe_sync": (c_int, [ctypes.c_void_p]),
     "mqe_kernel_path": (c_int, [ctypes.c_void_p]),
+    "mqe_set_canonical_interval": (c_int, [ctypes.c_void_p, c_int]),
     "mqe_num_walkers": (c_int, [ctypes.c_void_p, P(c_int)]),
     "mqe_num_couplings": (c_int, [ctypes.c_void_p, P(c_int)]),
     "mqe_read_scalars": (c_int, [ctypes.c_void_p, P(c_double), P(c_double), P(c_double), P(c_double)]),

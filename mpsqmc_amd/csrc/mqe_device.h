@@ -18,9 +18,32 @@ namespace mqe {
 // ---------------------------------------------------------------------------
 // reductions
 // ---------------------------------------------------------------------------
+// Data-parallel-primitive (DPP) lane exchange of a double within rows of 16
+// lanes; CTRL is the DPP control word (quad_perm 0x00-0xFF, row_mirror 0x140,
+// row_half_mirror 0x141).  All lanes of the row must be active.
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double x) {
+    const int lo = __double2loint(x), hi = __double2hiint(x);
+    const int lo2 = __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, true);
+    const int hi2 = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, true);
+    return __hiloint2double(hi2, lo2);
+}
+
+// sum over aligned groups of 8 lanes (xor 1, xor 2 inside quads, then the
+// mirrored half-row), every lane of the group receives the sum
+__device__ __forceinline__ double group8_sum(double v) {
+    v += dpp_d<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_d<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_d<0x141>(v);  // row_half_mirror
+    return v;
+}
+
+// sum over the wave64; every lane receives the total
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    v = group8_sum(v);
+    v += dpp_d<0x140>(v);  // row_mirror: the two 8-lane halves of a 16-lane row
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
     return v;
 }
 

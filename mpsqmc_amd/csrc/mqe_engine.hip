@@ -46,7 +46,7 @@ namespace mqe {
 static thread_local std::string g_err;
 void set_error(const char *msg) { g_err = msg ? msg : ""; }
 
-enum : int { F_PREP = 1, F_PROP = 2 };
+enum : int { F_PREP = 1, F_PROP = 2, F_CANON = 4 };
 
 // Read-only description of the system (device pointers).
 struct Sys {
@@ -1017,6 +1017,8 @@ struct mqe_engine {
     int last_walkers = 0;
     hipGraphExec_t graph_exec = nullptr;
     // per-launch timing of the walker kernel inside mqe_step / mqe_run
+    int canon_every = 4;   // tile path: FP64 LQ re-canonicalisation every this many steps
+    long prop_count = 0;
     bool timing = false;
     std::vector<hipEvent_t> tev;
     int tev_used = 0;
@@ -1085,6 +1087,10 @@ static void launch_walker(mqe_engine *e, const double *u, int step, int flags) {
 }
 
 static int launch_walker_any(mqe_engine *e, const double *u, int step, int flags) {
+    if (flags & F_PROP) {
+        if (e->canon_every <= 1 || (e->prop_count % e->canon_every) == e->canon_every - 1) flags |= F_CANON;
+        ++e->prop_count;
+    }
     if (e->tile32) {
         Pool P = make_pool(e, u, step);
         hipLaunchKernelGGL(t32::walker32_kernel, dim3(e->p.capacity), dim3(MQE_NT), t32::lds_bytes(), e->stream,
@@ -1202,7 +1208,7 @@ int mqe_create(const mqe_params *p, const mqe_mps *trial, int n_couplings, const
                     n_grid <= t32::MAXG_T32 && !(no32 && no32[0] == '1');
         if (e->tile32) {
             e->wflat = 2L * L * 1024;
-            e->eflat = (long)(L + 1) * 1024;
+            e->eflat = 2L * (L + 1) * 1024;  // R_k blocks, then the right norm environments M_k
         }
     }
 
@@ -1419,6 +1425,7 @@ int mqe_load_walkers(mqe_engine *e, int n, const int *vd, const double *data, co
         for (int sl = 0; sl < e->p.capacity; ++sl) {
             env[(size_t)sl * e->eflat] = 1.f;
             env[(size_t)sl * e->eflat + (size_t)L * 1024] = 1.f;
+            env[(size_t)sl * e->eflat + (size_t)(2 * L + 1) * 1024] = 1.f;  // M_L = 1
         }
         HIPCHK(hipMemcpy(e->d_Lenv, env.data(), sizeof(float) * env.size(), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(e->d_Renv, env.data(), sizeof(float) * env.size(), hipMemcpyHostToDevice));
@@ -1579,6 +1586,13 @@ int mqe_read_trace(mqe_engine *e, int first_step, int n_steps, double *out) {
 }
 
 int mqe_kernel_path(mqe_engine *e) { return (e && e->tile32) ? 1 : 0; }
+
+int mqe_set_canonical_interval(mqe_engine *e, int steps) {
+    if (!e || steps < 1) { set_error("mqe_set_canonical_interval: bad arguments"); return MQE_EINVAL; }
+    e->canon_every = steps;
+    e->prop_count = 0;
+    return MQE_OK;
+}
 
 int mqe_sync(mqe_engine *e) {
     if (!e) { set_error("mqe_sync: null engine"); return MQE_EINVAL; }

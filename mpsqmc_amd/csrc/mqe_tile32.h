@@ -40,8 +40,6 @@ enum : int {
     B_V = 12,    // 4 blocks (X after the pair tensor)
     NBLK = 16
 };
-constexpr int QLD = 65;                         // FP64 QR workspace leading dim
-constexpr int Q_TAU = 64 * QLD;                 // tau after the 64 x 65 matrix
 constexpr int MAXG_T32 = (2 * BLK * 4) / 16;    // grid points that fit PDF + CDF in B_W
 
 constexpr size_t lds_bytes() { return sizeof(float) * (size_t)NBLK * BLK; }
@@ -112,110 +110,128 @@ struct W32 {
 // FP64 Householder LQ of the center tensor (two LDS blocks, w_k x 2 w_{k+1}):
 // phi = Lmat Q with Q right-orthonormal.  Q overwrites the center blocks and
 // Lmat (lower triangular, w_k x w_k) goes to B_LM.  The transposed matrix
-// A = phi^T (m = 2 w_{k+1} rows, n = w_k columns) is factorised; row index
-// r = col + w_{k+1} s.  32 groups of 8 lanes, group g owns column g.
+// A = phi^T (m = 2 w_{k+1} <= 64 rows, n = w_k <= 32 columns) is factorised
+// with row index r = col + w_{k+1} s.  32 groups of 8 lanes: group g keeps
+// column g in registers (lane ln holds rows ln + 8t, t = 0..7); only the
+// reflector vectors and tau pass through LDS (V: 64 x 32 doubles, column j at
+// V + 64 j), one workgroup barrier per column.
 // ---------------------------------------------------------------------------
+
 __device__ void lq32(const W32 &C, int wk, int wn, float *center) {
     const int m = 2 * wn, n = wk;
-    double *A = (double *)C.blk(B_N), *tau = A + Q_TAU;
+    double *V = (double *)C.blk(B_N), *tau = V + 64 * 32;
     const int tid = threadIdx.x, g = tid >> 3, ln = tid & 7;
-    for (int e = tid; e < 64 * 32; e += MQE_NT) {
-        const int r = e & 63, c = e >> 6;
+    // this group's column of A = phi^T
+    double a[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        const int r = ln + 8 * t;
         double v = 0.0;
-        if (r < m && c < n) {
+        if (r < m && g < n) {
             const int s = r >= wn ? 1 : 0, col = r - s * wn;
-            v = (double)center[s * BLK + c + LD * col];
+            v = (double)center[s * BLK + g + LD * col];
         }
-        A[r + QLD * c] = v;
+        a[t] = v;
     }
-    __syncthreads();
-    // reflector of column 0 by group 0
-    auto make_reflector = [&](int j) {
-        // group j: column j, rows j..m-1
-        double *cj = A + QLD * j;
-        double ss = 0.0;
-        for (int i = j + 1 + ln; i < m; i += 8) ss += cj[i] * cj[i];
-        ss += __shfl_xor(ss, 1, 64);
-        ss += __shfl_xor(ss, 2, 64);
-        ss += __shfl_xor(ss, 4, 64);
-        const double alpha = cj[j];
-        double tj = 0.0;
+    // Householder reflector of the own column j (rows j..m-1): v_j = 1 implied,
+    // v_i (i > j) written to V, beta kept in a (row j), tau to LDS.
+    auto reflect = [&](int j) {
+        double ss = 0.0, alpha = 0.0;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int r = ln + 8 * t;
+            if (r > j) ss += a[t] * a[t];
+            if (r == j) alpha = a[t];
+        }
+        ss = group8_sum(ss);
+        alpha = group8_sum(alpha);
+        double tj = 0.0, sc = 0.0, beta = alpha;
         if (ss != 0.0) {
-            const double beta = -copysign(sqrt(alpha * alpha + ss), alpha);
+            beta = -copysign(sqrt(alpha * alpha + ss), alpha);
             tj = (beta - alpha) / beta;
-            const double sc = 1.0 / (alpha - beta);
-            for (int i = j + 1 + ln; i < m; i += 8) cj[i] *= sc;
-            // all 8 lanes read alpha above before lane 0 writes beta: lanes of a
-            // group are in one wave and execute in lockstep
-            if (ln == 0) cj[j] = beta;
+            sc = 1.0 / (alpha - beta);
+        }
+        double *vj = V + 64 * j;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int r = ln + 8 * t;
+            if (r > j) {
+                if (tj != 0.0) a[t] *= sc;
+                vj[r] = (tj != 0.0) ? a[t] : 0.0;
+            }
+            if (r == j) a[t] = beta;
         }
         if (ln == 0) tau[j] = tj;
     };
-    if (g == 0) make_reflector(0);
+    if (g == 0) reflect(0);
     __syncthreads();
     for (int j = 0; j < n; ++j) {
-        const double tj = tau[j];
-        const double *vj = A + QLD * j;
-        if (g > j && g < n && tj != 0.0) {
-            double *cg = A + QLD * g;
-            double s = 0.0;
-            for (int i = j + 1 + ln; i < m; i += 8) s += vj[i] * cg[i];
-            s += __shfl_xor(s, 1, 64);
-            s += __shfl_xor(s, 2, 64);
-            s += __shfl_xor(s, 4, 64);
-            s = tj * (s + cg[j]);
-            for (int i = j + ln; i < m; i += 8) cg[i] -= s * (i == j ? 1.0 : vj[i]);
+        if (g > j && g < n) {
+            const double tj = tau[j];
+            if (tj != 0.0) {
+                const double *vj = V + 64 * j;
+                double vr[8];
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    const int r = ln + 8 * t;
+                    vr[t] = (r > j && r < m) ? vj[r] : (r == j ? 1.0 : 0.0);
+                }
+                double s = 0.0;
+#pragma unroll
+                for (int t = 0; t < 8; ++t) s += vr[t] * a[t];
+                s = tj * group8_sum(s);
+#pragma unroll
+                for (int t = 0; t < 8; ++t) a[t] -= s * vr[t];
+            }
+            if (g == j + 1) reflect(j + 1);
         }
-        if (g == j + 1 && g < n) make_reflector(j + 1);
         __syncthreads();
     }
-    // Lmat = R^T (lower), from the upper triangle of A
+    // Lmat = R^T (lower): R(j, g) for j <= g is held in group g, row j
     float *LM = C.blk(B_LM);
-    for (int e = tid; e < 32 * 32; e += MQE_NT) {
-        const int a = e & 31, b = e >> 5;
-        LM[a + LD * b] = (a < n && b <= a) ? (float)A[b + QLD * a] : 0.f;
+    for (int e = tid; e < BLK; e += MQE_NT) LM[e] = 0.f;
+    __syncthreads();
+    if (g < n) {
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int r = ln + 8 * t;
+            if (r <= g) LM[g + LD * r] = (float)a[t];  // Lmat(g, r) = R(r, g)
+        }
     }
-    // Q column c = H_0 ... H_c e_c, group c; result written transposed into the center
+    // Q column c = H_0 H_1 ... H_c e_c, group c; written transposed into the center
     if (g < n) {
         const int c = g;
         double q[8];
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-            const int i = ln + 8 * t;
-            q[t] = (i == c) ? 1.0 : 0.0;
-        }
+        for (int t = 0; t < 8; ++t) q[t] = (ln + 8 * t == c) ? 1.0 : 0.0;
         for (int k = c; k >= 0; --k) {
             const double tk = tau[k];
             if (tk == 0.0) continue;
-            const double *vk = A + QLD * k;
+            const double *vk = V + 64 * k;
+            double vr[8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int r = ln + 8 * t;
+                vr[t] = (r > k && r < m) ? vk[r] : (r == k ? 1.0 : 0.0);
+            }
             double s = 0.0;
 #pragma unroll
-            for (int t = 0; t < 8; ++t) {
-                const int i = ln + 8 * t;
-                if (i >= k && i < m) s += (i == k ? 1.0 : vk[i]) * q[t];
-            }
-            s += __shfl_xor(s, 1, 64);
-            s += __shfl_xor(s, 2, 64);
-            s += __shfl_xor(s, 4, 64);
-            s *= tk;
+            for (int t = 0; t < 8; ++t) s += vr[t] * q[t];
+            s = tk * group8_sum(s);
 #pragma unroll
-            for (int t = 0; t < 8; ++t) {
-                const int i = ln + 8 * t;
-                if (i >= k && i < m) q[t] -= s * (i == k ? 1.0 : vk[i]);
-            }
+            for (int t = 0; t < 8; ++t) q[t] -= s * vr[t];
         }
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
-            const int i = ln + 8 * t;
-            if (i < m) {
-                const int s = i >= wn ? 1 : 0, col = i - s * wn;
+            const int r = ln + 8 * t;
+            if (r < m) {
+                const int s = r >= wn ? 1 : 0, col = r - s * wn;
                 center[s * BLK + c + LD * col] = (float)q[t];
             }
         }
     }
     __syncthreads();
 }
-
 
 // ---------------------------------------------------------------------------
 // the walker kernel (flags: F_PREP or F_PROP), h = 0, couplings (i, i+1)
@@ -234,9 +250,10 @@ __device__ __forceinline__ void apply2(float *b, const double *O) {
     }
 }
 
-// <trial|phi> with the walker's site-0 tensor in LDS block pair c0 and R_1 in B_R;
-// returns {overlap, squared norm of site 0}
-__device__ void site0_32(const W32 &C, const float *c0, double *ov, double *n2) {
+// <trial|phi> with the walker's site-0 tensor in LDS block pair c0 and R_1 in
+// B_R, and the squared norm sum_s <phi_0[s], phi_0[s] M_1> with the right norm
+// environment M_1 (Mb; nullptr = identity, i.e. sites > 0 right-orthonormal).
+__device__ void site0_32(const W32 &C, const float *c0, const float *Mb, double *ov, double *n2) {
     const float *R1 = C.blk(B_R);
     double a = 0.0, b = 0.0;
     for (int e = threadIdx.x; e < 2 * 32 * 32; e += MQE_NT) {
@@ -244,7 +261,8 @@ __device__ void site0_32(const W32 &C, const float *c0, double *ov, double *n2) 
         const double t = (double)C.tn(0, s)[0 + 32 * al];
         const double p = (double)c0[s * BLK + 0 + LD * be];
         a += t * p * (double)R1[al + LD * be];
-        if (al == 0) b += p * p;
+        if (Mb) b += (double)c0[s * BLK + 0 + LD * al] * (double)Mb[al + LD * be] * p;
+        else if (al == 0) b += p * p;
     }
     double v[2] = {a, b};
     block_sum_n<2>(v, C.red);
@@ -252,16 +270,103 @@ __device__ void site0_32(const W32 &C, const float *c0, double *ov, double *n2) 
     *n2 = v[1];
 }
 
+// global identity (restricted to w) into a padded block
+__device__ __forceinline__ void g_identity(float *g, int w) {
+    for (int e = threadIdx.x; e < GB; e += MQE_NT) {
+        const int r = e & 31, c = e >> 5;
+        g[e] = (r == c && r < w) ? 1.f : 0.f;
+    }
+}
+
 // Energy sweep (right -> left) from the center phi_{L-1} (LDS pair pc); L_k in
 // global memory.  Returns <trial|H|phi>; *Ou = <trial|phi>; normalises site 0.
-__device__ double energy_sweep32(const W32 &C, int pc, double *On, double *Ou) {
+__device__ double energy_sweep32(const W32 &C, int pc, double *On, double *Ou, bool canon) {
     const Sys &S = *C.S;
     const int L = S.L;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     __shared__ double Mq[16];
     set_unit(C.blk(B_R));
+    set_unit(C.blk(B_LM));  // M_L = 1 (light sweep)
     double etot = 0.0;
     int pj = pc == B_BUF0 ? B_BUF1 : B_BUF0;
+    if (!canon) {
+        // light sweep: no re-orthonormalisation; right environments R_k and
+        // right norm environments M_k = sum_t phi_k[t] M_{k+1} phi_k[t]^T.
+        // The last site's propagated tensor is only in LDS: store it.
+        l2g(C.blk(pc), C.ph(L - 1, 0));
+        l2g(C.blk(pc) + BLK, C.ph(L - 1, 1));
+        for (int k = L - 1; k >= 1; --k) {
+            float *cen = C.blk(pc), *nxt = C.blk(pj);
+            g2l(C.ph(k - 1, 0), nxt);
+            g2l(C.ph(k - 1, 1), nxt + BLK);
+            g2l(C.Lg + (size_t)(k - 1) * GB, C.blk(B_L));
+            __syncthreads();
+            {  // stage 1: W[t] = R_{k+1} phi_k[t]^T (waves 0,1); X[t] = phi_k[t] M_{k+1} (waves 2,3)
+                v16f acc = zero16();
+                if (w < 2) {
+                    mma<false, true>(acc, C.blk(B_R), LD, cen + w * BLK, LD);
+                    store(acc, C.blk(B_W) + w * BLK, LD);
+                } else {
+                    mma<false, false>(acc, cen + (w - 2) * BLK, LD, C.blk(B_LM), LD);
+                    store(acc, C.blk(B_Y) + (w - 2) * BLK, LD);
+                }
+            }
+            __syncthreads();
+            {  // stage 2: V(t',t) = T_k[t'] W[t]; M_k = sum_t X[t] phi_k[t]^T (wave 0)
+                v16f acc = zero16();
+                mma<false, false>(acc, C.tn(k, w >> 1), 32, C.blk(B_W) + (w & 1) * BLK, LD);
+                store(acc, C.blk(B_V) + w * BLK, LD);
+                if (w == 0) {
+                    v16f m = zero16();
+                    mma<false, true>(m, C.blk(B_Y), LD, cen, LD);
+                    mma<false, true>(m, C.blk(B_Y) + BLK, LD, cen + BLK, LD);
+                    store(m, C.blk(B_LM), LD);
+                }
+            }
+            __syncthreads();
+            {  // stage 3: R_k; Y[s] = L_{k-1} phi_{k-1}[s] (waves 0,1)
+                const float *V = C.blk(B_V);
+                float *R = C.blk(B_R);
+                for (int e = threadIdx.x; e < BLK; e += MQE_NT) R[e] = V[e] + V[3 * BLK + e];
+                if (w < 2) {
+                    v16f y = zero16();
+                    mma<false, false>(y, C.blk(B_L), LD, nxt + w * BLK, LD);
+                    store(y, C.blk(B_Y) + w * BLK, LD);
+                }
+            }
+            __syncthreads();
+            {  // stage 4: bond energy of (k-1, k); environments to global memory
+                const float *V = C.blk(B_V);
+                v16f g = zero16();
+                mma<false, false>(g, C.tt(k - 1, w >> 1), 32, C.blk(B_Y) + (w & 1) * BLK, LD);
+                const int sp = w >> 1, ss = w & 1;
+                for (int q = 0; q < 4; ++q) {
+                    const double d = dot_tile(g, V + q * BLK, LD);
+                    if (lane == 0) Mq[(sp + 2 * (q >> 1)) * 4 + (ss + 2 * (q & 1))] = d;
+                }
+                l2g(C.blk(B_R), C.Rg + (size_t)k * GB);
+                l2g(C.blk(B_LM), C.Rg + (size_t)(L + 1 + k) * GB);
+            }
+            __syncthreads();
+            double e = 0.0;
+            for (int q = 0; q < 16; ++q) e += S.SS[q] * Mq[q];
+            etot += S.Jc[k - 1] * e;
+            int t = pc; pc = pj; pj = t;
+        }
+        double ov, n2;
+        float *c0 = C.blk(pc);
+        site0_32(C, c0, C.blk(B_LM), &ov, &n2);
+        const double nrm = sqrt(n2);
+        double O = ov / nrm, f = 1.0 / nrm;
+        if (O < 0.0) { O = -O; f = -f; }
+        scale_blocks(c0, 2, f);
+        __syncthreads();
+        l2g(c0, C.ph(0, 0));
+        l2g(c0 + BLK, C.ph(0, 1));
+        *On = O;
+        *Ou = ov;
+        return etot;
+    }
     for (int k = L - 1; k >= 1; --k) {
         float *cen = C.blk(pc), *nxt = C.blk(pj);
         g2l(C.ph(k - 1, 0), nxt);
@@ -312,6 +417,7 @@ __device__ double energy_sweep32(const W32 &C, int pc, double *On, double *Ou) {
         }
         __syncthreads();
         l2g(C.blk(B_R), C.Rg + (size_t)k * GB);
+        g_identity(C.Rg + (size_t)(L + 1 + k) * GB, C.wv[k]);  // M_k = 1: sites >= k right-orthonormal
         double e = 0.0;
         for (int q = 0; q < 16; ++q) e += S.SS[q] * Mq[q];
         etot += S.Jc[k - 1] * e;
@@ -320,7 +426,7 @@ __device__ double energy_sweep32(const W32 &C, int pc, double *On, double *Ou) {
     }
     double ov, n2;
     float *c0 = C.blk(pc);
-    site0_32(C, c0, &ov, &n2);
+    site0_32(C, c0, nullptr, &ov, &n2);
     const double nrm = sqrt(n2);
     double O = ov / nrm, f = 1.0 / nrm;
     if (O < 0.0) { O = -O; f = -f; }
@@ -387,7 +493,7 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
         g2l(C.ph(L - 1, 1), cen + BLK);
         __syncthreads();
         double On, Ou;
-        const double e = energy_sweep32(C, B_BUF0, &On, &Ou);
+        const double e = energy_sweep32(C, B_BUF0, &On, &Ou, true);
         if (threadIdx.x == 0) {
             P.EL[slot] = e / O;
             P.O[slot] = On;
@@ -408,6 +514,7 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
         g2l(C.ph(j, 0), nxt);
         g2l(C.ph(j, 1), nxt + BLK);
         g2l(C.Rg + (size_t)(j + 1) * GB, C.blk(B_R));
+        g2l(C.Rg + (size_t)(L + 2 + j) * GB, C.blk(B_LM));  // right norm environment M_{j+1}
         __syncthreads();
         // stage A: Y[s] = L_i phi_i[s] (waves 0,1); W[t] = R_{j+1} phi_j[t]^T (waves 2,3)
         {
@@ -521,6 +628,11 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
                 v16f x = zero16();
                 mma<false, false>(x, C.blk(B_N), LD, cen + w * BLK, LD);
                 store(x, C.blk(B_V) + w * BLK, LD);
+            } else {
+                // Z1[t] = phi'_j[t] M_{j+1}
+                v16f z1 = zero16();
+                mma<false, false>(z1, nxt + (w - 2) * BLK, LD, C.blk(B_LM), LD);
+                store(z1, C.blk(B_V) + w * BLK, LD);
             }
         }
         __syncthreads();
@@ -533,10 +645,10 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
         }
         l2g(C.blk(B_L), C.Lg + (size_t)(i + 1) * GB);
         __syncthreads();
-        // stage H: squared norm = sum_t <phi'_j[t], N_{i+1} phi'_j[t]> (right part right-orthonormal)
+        // stage H: squared norm = sum_t <phi'_j[t], N_{i+1} phi'_j[t] M_{j+1}>
         if (w < 2) {
             v16f z = zero16();
-            mma<false, false>(z, C.blk(B_N), LD, nxt + w * BLK, LD);
+            mma<false, false>(z, C.blk(B_N), LD, C.blk(B_V) + (2 + w) * BLK, LD);
             const double d = dot_tile(z, nxt + w * BLK, LD);
             if (lane == 0) scal[1 + w] = d;
         }
@@ -549,7 +661,7 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
     }
     // ---------------- local energy (Walker.cpp:72-77) ----------------
     double On, Ou;
-    const double e = energy_sweep32(C, pc, &On, &Ou);
+    const double e = energy_sweep32(C, pc, &On, &Ou, (flags & F_CANON) != 0);
     if (threadIdx.x == 0) {
         P.w[slot] = wt;
         P.O[slot] = On;

@@ -125,6 +125,9 @@ class Engine:
     def run(self, first_step, n_steps, use_graph=False):
         check(lib().mqe_run(self.h, first_step, n_steps, 1 if use_graph else 0))
 
+    def set_canonical_interval(self, steps):
+        check(lib().mqe_set_canonical_interval(self.h, steps))
+
     @property
     def path(self):
         """'tile32' (FP32 MFMA path) or 'general'."""

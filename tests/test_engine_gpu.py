@@ -87,8 +87,11 @@ def test_prepare_energy_fp64():
     assert rel(sc["EL"], EL) < FP64_TOL
 
 
-def _run_parity(system, D, n, seed, steps, precision="fp64", tol=FP64_TOL, uneven=False, check_states=True):
+def _run_parity(system, D, n, seed, steps, precision="fp64", tol=FP64_TOL, uneven=False, check_states=True,
+                canon=None):
     orc, eng, _ = _pair(system, D, n, seed, precision, uneven)
+    if canon is not None:
+        eng.set_canonical_interval(canon)
     E_e = eng.prepare()
     E_o, _, _ = orc.energy(1, dofluct=False)
     assert abs(E_e - E_o) <= tol * abs(E_o)
@@ -221,9 +224,11 @@ def test_tile32_path_selected():
     eng.close()
 
 
-def test_tile32_parity_chain16_fp32():
-    # FP32 MFMA path against the FP64 oracle: tolerance 2e-4 relative
-    _run_parity(systems.chain(16), 16, 8, 17, steps=2, precision="fp32", tol=2e-4, check_states=False)
+@pytest.mark.parametrize("canon", [1, 2, 4])
+def test_tile32_parity_chain16_fp32(canon):
+    # FP32 MFMA path against the FP64 oracle: tolerance 2e-4 relative; the LQ
+    # re-canonicalisation every `canon` steps changes the gauge only
+    _run_parity(systems.chain(16), 16, 8, 17, steps=5, precision="fp32", tol=2e-4, check_states=False, canon=canon)
 
 
 def test_tile32_parity_chain32_d32_fp32():
