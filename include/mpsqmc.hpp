@@ -105,6 +105,7 @@ class MPSstate {
     void CompressState(int truncD);
     void CompressState() { CompressState(Dtrunc_); }
     const std::vector<int> &bonds() const { return vd_; }
+    void assign(const std::vector<int> &vd, const std::vector<double> &data);
     const std::vector<double> &data() const { return data_; }
     std::vector<double> &data() { return data_; }
     friend std::ostream &operator<<(std::ostream &os, const MPSstate &m);
@@ -114,6 +115,22 @@ class MPSstate {
     int L_, Dtrunc_, d_;
     std::vector<int> vd_;
     std::vector<double> data_;
+};
+
+// DMRG.h:11-60: two-site DMRG for the trial state; Solve() replaces the
+// state's tensors by the optimised, left-normalised ones (bond profile of its
+// Dtrunc) and returns the energy.  The random start is drawn from a seeded
+// stream instead of the state's own random tensors.
+class DMRG {
+  public:
+    DMRG(MPSstate *MPStrial, const HeisenbergMPO *theMPO, unsigned int seed = 20131015u)
+        : mps_(MPStrial), mpo_(theMPO), seed_(seed) {}
+    double Solve(int maxSweeps = 30, double tol = 1e-10);
+
+  private:
+    MPSstate *mps_;
+    const HeisenbergMPO *mpo_;
+    unsigned int seed_;
 };
 
 // Walker.h:13-52 (a host copy of one GPU walker)
@@ -137,10 +154,14 @@ struct StepRecord {
     double projectedEnergy, targetEnergy, fluctMetric, sumWeights;
 };
 
-// MPSQMC2.h:21,27 with the trial given (the DMRG-running constructor is not provided)
+// MPSQMC2.h:18,21,27
 class MPSQMC2 {
   public:
     enum class Rng { Host, Device };
+    // trial by DMRG with bond dimension DT (MPSQMC2.cpp:225-236)
+    MPSQMC2(HeisenbergMPO *theMPO, GridGenerator *theGrid, Random *RN, int DT, int DW, int Nwalkers, double dtau,
+            int precision = MQE_PREC_FP64, Rng rng = Rng::Host);
+    // trial given
     MPSQMC2(HeisenbergMPO *theMPO, GridGenerator *theGrid, Random *RN, MPSstate *Psi0_in, int DW, int Nwalkers,
             double dtau, int precision = MQE_PREC_FP64, Rng rng = Rng::Host);
     ~MPSQMC2();
@@ -155,11 +176,13 @@ class MPSQMC2 {
     void setVerbose(bool v) { verbose_ = v; }
     const std::vector<StepRecord> &history() const { return history_; }
     double energyAtStart() const { return E_start_; }
+    double trialEnergy() const { return E_dmrg_; }  // DMRG energy (DT constructor), else 0
     int nWalkers() const;
     Walker walker(int i) const;
     mqe_engine *engine() { return eng_; }
 
   private:
+    void setup(MPSstate *Psi0_in, int precision);
     double PropagateSeparately();
     void SeparatePopulationControl(double scaling);
     double EnergyFunctionAndHistory(int step, double *projectedEnergy, bool doFluctuationMetric = true);
@@ -179,6 +202,7 @@ class MPSQMC2 {
     bool started_ = false;
     int step_offset_ = 0;
     double E_start_ = 0.0;
+    double E_dmrg_ = 0.0;
     std::vector<StepRecord> history_;
 };
 

@@ -171,6 +171,16 @@ int mqe_left_normalize(int L, int d, const int *vd, double *data, double *norm);
  * the tensors (capacity: flat size of the input). */
 int mqe_compress(int L, int d, const int *vd, const double *data, int Dt, int *vd_out, double *out);
 
+/* Trial wave function by two-site DMRG (DMRG.cpp:50-100 as called from
+ * MPSQMC2::SetupTrial, MPSQMC2.cpp:225-236): ground state of
+ * H = sum_c J[c] S_first[c].S_second[c] - field sum_i S^z_i in the bond profile
+ * mqe_bond_dims(L, D, d), from a random start drawn from std::mt19937(seed),
+ * sweeping until the energy changes by less than tol (or max_sweeps).  The
+ * result is left-normalized.  vd_out: L+1 ints; data_out: the flat size of
+ * that profile; *energy: the variational energy.  d must be 2. */
+int mqe_dmrg(int L, int d, int n_couplings, const int *first, const int *second, const double *J, double field,
+             int D, int max_sweeps, double tol, unsigned int seed, int *vd_out, double *data_out, double *energy);
+
 /* ---- host random stream (the build's RNG contract) ------------------------- */
 /* Random::rand (Random.cpp:49-59) with a fixed seed: one std::mt19937 stream,
  * u = x * 2^-32.  peek fills out without consuming; discard consumes k draws. */

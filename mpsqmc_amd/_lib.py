@@ -78,6 +78,8 @@ SIGNATURES = {
     "mqe_bond_dims": (None, [c_int, c_int, c_int, P(c_int)]),
     "mqe_trotter_tables": (c_int, [c_int, c_double, c_double, P(c_double), P(c_double), P(c_double)]),
     "mqe_marsaglia_grid": (c_int, [c_int, c_int, P(c_double), P(c_int)]),
+    "mqe_dmrg": (c_int, [c_int, c_int, c_int, P(c_int), P(c_int), P(c_double), c_double, c_int, c_int, c_double,
+                         ctypes.c_uint, P(c_int), P(c_double), P(c_double)]),
     "mqe_left_normalize": (c_int, [c_int, c_int, P(c_int), P(c_double), P(c_double)]),
     "mqe_compress": (c_int, [c_int, c_int, P(c_int), P(c_double), c_int, P(c_int), P(c_double)]),
     "mqe_rng_create": (c_int, [ctypes.c_uint, P(ctypes.c_void_p)]),

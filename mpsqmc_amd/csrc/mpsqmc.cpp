@@ -222,11 +222,61 @@ std::ostream &operator<<(std::ostream &os, const MPSstate &m) {  // MPSstate.cpp
     return os;
 }
 
+void MPSstate::assign(const std::vector<int> &vd, const std::vector<double> &data) {
+    if ((int)vd.size() != L_ + 1) throw std::invalid_argument("MPSstate::assign: bond count");
+    vd_ = vd;
+    data_.assign(data.begin(), data.begin() + offset(L_));
+}
+
+// ---------------------------------------------------------------- DMRG
+double DMRG::Solve(int maxSweeps, double tol) {
+    const int L = mpo_->gLength(), d = mpo_->gPhys_d();
+    std::vector<int> first, second;
+    std::vector<double> Jc;
+    for (int i = 0; i < L - 1; ++i)
+        for (int j = i + 1; j < L; ++j)
+            if (mpo_->gCoupling(i, j) != 0.0) {
+                first.push_back(i);
+                second.push_back(j);
+                Jc.push_back(mpo_->gCoupling(i, j));
+            }
+    std::vector<int> vd(L + 1);
+    mqe_bond_dims(L, mps_->gDtrunc(), d, vd.data());
+    long n = 0;
+    for (int k = 0; k < L; ++k) n += (long)vd[k] * vd[k + 1] * d;
+    std::vector<double> data(n);
+    double E = 0.0;
+    ck(mqe_dmrg(L, d, (int)first.size(), first.data(), second.data(), Jc.data(), mpo_->gField(), mps_->gDtrunc(),
+                maxSweeps, tol, seed_, vd.data(), data.data(), &E),
+       "DMRG::Solve");
+    mps_->assign(vd, data);
+    return E;
+}
+
 // ---------------------------------------------------------------- MPSQMC2
+MPSQMC2::MPSQMC2(HeisenbergMPO *theMPO, GridGenerator *theGrid, Random *RN, int DT, int DW, int Nwalkers,
+                 double dtau, int precision, Rng rng)
+    : theMPO_(theMPO), theGrid_(theGrid), RN_(RN), DW_(DW), totalNDesiredWalkers_(Nwalkers),
+      totalNCurrentWalkers_(Nwalkers), dtau_(dtau), rng_(rng) {
+    // SetupTrial(true) (MPSQMC2.cpp:225-236): DMRG, then left-normalise
+    MPSstate trial(theMPO->gLength(), DT, theMPO->gPhys_d(), nullptr);
+    DMRG solver(&trial, theMPO);
+    E_dmrg_ = solver.Solve();
+    setup(&trial, precision);
+}
+
 MPSQMC2::MPSQMC2(HeisenbergMPO *theMPO, GridGenerator *theGrid, Random *RN, MPSstate *Psi0_in, int DW,
                  int Nwalkers, double dtau, int precision, Rng rng)
     : theMPO_(theMPO), theGrid_(theGrid), RN_(RN), DW_(DW), totalNDesiredWalkers_(Nwalkers),
       totalNCurrentWalkers_(Nwalkers), dtau_(dtau), rng_(rng) {
+    setup(Psi0_in, precision);
+}
+
+void MPSQMC2::setup(MPSstate *Psi0_in, int precision) {
+    HeisenbergMPO *theMPO = theMPO_;
+    GridGenerator *theGrid = theGrid_;
+    const int DW = DW_, Nwalkers = totalNDesiredWalkers_;
+    const double dtau = dtau_;
     const int L = theMPO->gLength(), d = theMPO->gPhys_d();
     if (theGrid->gDim() != d * d) throw std::invalid_argument("MPSQMC2: grid dimension must be d*d");
     theTrotter_ = new TrotterHeisenberg(theMPO, dtau);

@@ -3,12 +3,13 @@
 // instead of hard-coded values).
 //
 //   mpsqmc_walk --L 20 --trial debug.mps --DW 2 --N 1000 --dtau 0.01 --steps 100
-//               [--D 4 (random trial)] [--seed 1] [--fp32] [--device-rng] [--trace file]
-//               [--snake ROWSxCOLS] [--field h]
+//               [--D 4 (random trial)] [--dmrg (trial by DMRG with bond dimension D)]
+//               [--seed 1] [--fp32] [--device-rng] [--trace file] [--snake ROWSxCOLS] [--field h]
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
+#include <memory>
 #include <string>
 
 #include "mpsqmc.hpp"
@@ -19,7 +20,7 @@ int main(int argc, char **argv) {
     int L = 16, D = 8, DW = -1, N = 128, steps = 10, rows = 0, cols = 0;
     unsigned seed = 20131015u;
     double dtau = 0.01, field = 0.0;
-    bool fp32 = false, devrng = false;
+    bool fp32 = false, devrng = false, dmrg = false;
     std::string trial_file, trace;
     for (int a = 1; a < argc; ++a) {
         std::string k = argv[a];
@@ -39,6 +40,7 @@ int main(int argc, char **argv) {
         else if (k == "--trace") trace = next();
         else if (k == "--fp32") fp32 = true;
         else if (k == "--device-rng") devrng = true;
+        else if (k == "--dmrg") dmrg = true;
         else if (k == "--snake") {
             if (std::sscanf(next(), "%dx%d", &rows, &cols) != 2) { std::fprintf(stderr, "--snake RxC\n"); return 2; }
             L = rows * cols;
@@ -63,8 +65,12 @@ int main(int argc, char **argv) {
         GridGenerator grid(4);
         grid.FillMarsaglia(4);
         if (DW < 0) DW = trial->gDtrunc();
-        MPSQMC2 qmc(&mpo, &grid, &RN, trial, DW, N, dtau, fp32 ? MQE_PREC_FP32 : MQE_PREC_FP64,
-                    devrng ? MPSQMC2::Rng::Device : MPSQMC2::Rng::Host);
+        const int prec = fp32 ? MQE_PREC_FP32 : MQE_PREC_FP64;
+        const MPSQMC2::Rng rk = devrng ? MPSQMC2::Rng::Device : MPSQMC2::Rng::Host;
+        std::unique_ptr<MPSQMC2> q(dmrg ? new MPSQMC2(&mpo, &grid, &RN, D, DW, N, dtau, prec, rk)
+                                        : new MPSQMC2(&mpo, &grid, &RN, trial, DW, N, dtau, prec, rk));
+        MPSQMC2 &qmc = *q;
+        if (dmrg) std::printf("E_dmrg %.15g\n", qmc.trialEnergy());
         if (!trace.empty()) qmc.setTraceFile(trace);
         qmc.Walk(steps);
         std::printf("E_start %.15g\n", qmc.energyAtStart());

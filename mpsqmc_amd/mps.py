@@ -147,3 +147,24 @@ def to_dense(vd, data, d=2):
         A = site_tensor(vd, data, k, d)  # (s, l, r)
         psi = np.einsum("pl,slr->psr", psi, A).reshape(-1, A.shape[2])
     return psi[:, 0].copy()
+
+
+def dmrg(system, D, max_sweeps=20, tol=1e-10, seed=20131015):
+    """Trial state by two-site DMRG (DMRG.cpp:50-100, as MPSQMC2::SetupTrial
+    uses it): ground state of the system's Hamiltonian in the MPSstate bond
+    profile of D, left-normalized.  Returns (vd, data, energy)."""
+    f, s, Jc = system.couplings()
+    L = system.L
+    f = np.ascontiguousarray(f, dtype=np.int32)
+    s = np.ascontiguousarray(s, dtype=np.int32)
+    Jc = np.ascontiguousarray(Jc, dtype=np.float64)
+    vd = np.asarray(bond_dims(L, D, system.d), dtype=np.int32)
+    out = np.zeros(flat_size(vd, system.d))
+    vo = np.zeros(L + 1, dtype=np.int32)
+    E = ctypes.c_double()
+    ip = ctypes.POINTER(ctypes.c_int)
+    dp = ctypes.POINTER(ctypes.c_double)
+    check(lib().mqe_dmrg(L, system.d, len(f), f.ctypes.data_as(ip), s.ctypes.data_as(ip), Jc.ctypes.data_as(dp),
+                         float(system.h), int(D), int(max_sweeps), float(tol), int(seed), vo.ctypes.data_as(ip),
+                         out.ctypes.data_as(dp), ctypes.byref(E)))
+    return vo, out, E.value
