@@ -47,6 +47,26 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// value of v in lane l (l wave-uniform)
+__device__ __forceinline__ double readlane_d(double v, int l) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
+
+// exclusive prefix sum over the lanes of the wave64 (lane 0 receives 0)
+__device__ __forceinline__ double wave_exclusive_scan(double v) {
+    const int lane = threadIdx.x & 63;
+    double s = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const double o = __shfl_up(s, off, 64);
+        if (lane >= off) s += o;
+    }
+    const double e = __shfl_up(s, 1, 64);
+    return lane == 0 ? 0.0 : e;
+}
+
 // Sum over the workgroup; every thread receives the total.  sh: >= MQE_WAVES doubles.
 __device__ __forceinline__ double block_sum(double v, double *sh) {
     v = wave_sum(v);

@@ -36,11 +36,11 @@ enum : int {
     B_LM = 6,    // Lmat of the LQ
     B_N = 7,     // left norm environment
     B_Y = 8,     // 2 blocks
-    B_W = 10,    // 2 blocks (also PDF/CDF scratch during sampling)
+    B_W = 10,    // 2 blocks
     B_V = 12,    // 4 blocks (X after the pair tensor)
     NBLK = 16
 };
-constexpr int MAXG_T32 = (2 * BLK * 4) / 16;    // grid points that fit PDF + CDF in B_W
+constexpr int MAXG_T32 = 256;                    // grid points: four per lane of one wave
 
 constexpr size_t lds_bytes() { return sizeof(float) * (size_t)NBLK * BLK; }
 
@@ -61,6 +61,26 @@ __device__ __forceinline__ void mma(v16f &acc, const float *A, int lda, const fl
         const float a = TA ? A[k + lda * r] : A[r + lda * k];
         const float b = TB ? B[r + ldb * k] : B[k + ldb * r];
         acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+    }
+}
+
+// A-operand fragment of a 32 x 32 block stored column-major with ld 32 (the
+// padded trial blocks in global memory): a[q] = A(r, 2q + h).  Loaded ahead of
+// the product so that the global-memory latency overlaps other work.
+typedef float frag16[16];
+__device__ __forceinline__ void load_frag(frag16 &a, const float *A) {
+    const int l = threadIdx.x & 63, r = l & 31, h = l >> 5;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) a[q] = A[r + 32 * (2 * q + h)];
+}
+
+// acc += A B with A given as a register fragment (load_frag) and B in LDS
+__device__ __forceinline__ void mma_frag(v16f &acc, const frag16 &a, const float *B, int ldb) {
+    const int l = threadIdx.x & 63, r = l & 31, h = l >> 5;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const float b = B[(2 * q + h) + ldb * r];
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[q], b, acc, 0, 0, 0);
     }
 }
 
@@ -297,6 +317,9 @@ __device__ double energy_sweep32(const W32 &C, int pc, double *On, double *Ou, b
         l2g(C.blk(pc) + BLK, C.ph(L - 1, 1));
         for (int k = L - 1; k >= 1; --k) {
             float *cen = C.blk(pc), *nxt = C.blk(pj);
+            frag16 fv, fg;  // trial operands of stages 2 and 4
+            load_frag(fv, C.tn(k, w >> 1));
+            load_frag(fg, C.tt(k - 1, w >> 1));
             g2l(C.ph(k - 1, 0), nxt);
             g2l(C.ph(k - 1, 1), nxt + BLK);
             g2l(C.Lg + (size_t)(k - 1) * GB, C.blk(B_L));
@@ -312,40 +335,46 @@ __device__ double energy_sweep32(const W32 &C, int pc, double *On, double *Ou, b
                 }
             }
             __syncthreads();
-            {  // stage 2: V(t',t) = T_k[t'] W[t]; M_k = sum_t X[t] phi_k[t]^T (wave 0)
+            {  // stage 2: V(t',t) = T_k[t'] W[t]
                 v16f acc = zero16();
-                mma<false, false>(acc, C.tn(k, w >> 1), 32, C.blk(B_W) + (w & 1) * BLK, LD);
+                mma_frag(acc, fv, C.blk(B_W) + (w & 1) * BLK, LD);
                 store(acc, C.blk(B_V) + w * BLK, LD);
-                if (w == 0) {
-                    v16f m = zero16();
-                    mma<false, true>(m, C.blk(B_Y), LD, cen, LD);
-                    mma<false, true>(m, C.blk(B_Y) + BLK, LD, cen + BLK, LD);
-                    store(m, C.blk(B_LM), LD);
-                }
             }
             __syncthreads();
-            {  // stage 3: R_k; Y[s] = L_{k-1} phi_{k-1}[s] (waves 0,1)
+            {  // stage 3: R_k = V(0,0) + V(1,1); Y[s] = L_{k-1} phi_{k-1}[s] (waves 0,1, into B_W);
+               // the two terms X[t] phi_k[t]^T of M_k (waves 2,3, into B_LM and B_N)
                 const float *V = C.blk(B_V);
                 float *R = C.blk(B_R);
                 for (int e = threadIdx.x; e < BLK; e += MQE_NT) R[e] = V[e] + V[3 * BLK + e];
+                v16f acc = zero16();
                 if (w < 2) {
-                    v16f y = zero16();
-                    mma<false, false>(y, C.blk(B_L), LD, nxt + w * BLK, LD);
-                    store(y, C.blk(B_Y) + w * BLK, LD);
+                    mma<false, false>(acc, C.blk(B_L), LD, nxt + w * BLK, LD);
+                    store(acc, C.blk(B_W) + w * BLK, LD);
+                } else {
+                    mma<false, true>(acc, C.blk(B_Y) + (w - 2) * BLK, LD, cen + (w - 2) * BLK, LD);
+                    store(acc, C.blk(w == 2 ? B_LM : B_N), LD);
                 }
             }
             __syncthreads();
-            {  // stage 4: bond energy of (k-1, k); environments to global memory
+            {  // stage 4: bond energy of (k-1, k); M_k = sum of its two terms; environments to global memory
                 const float *V = C.blk(B_V);
                 v16f g = zero16();
-                mma<false, false>(g, C.tt(k - 1, w >> 1), 32, C.blk(B_Y) + (w & 1) * BLK, LD);
+                mma_frag(g, fg, C.blk(B_W) + (w & 1) * BLK, LD);
                 const int sp = w >> 1, ss = w & 1;
                 for (int q = 0; q < 4; ++q) {
                     const double d = dot_tile(g, V + q * BLK, LD);
                     if (lane == 0) Mq[(sp + 2 * (q >> 1)) * 4 + (ss + 2 * (q & 1))] = d;
                 }
                 l2g(C.blk(B_R), C.Rg + (size_t)k * GB);
-                l2g(C.blk(B_LM), C.Rg + (size_t)(L + 1 + k) * GB);
+                float *Mk = C.blk(B_LM);
+                const float *Mb = C.blk(B_N);
+                float *mg = C.Rg + (size_t)(L + 1 + k) * GB;
+                for (int e = threadIdx.x; e < GB; e += MQE_NT) {
+                    const int o = (e & 31) + LD * (e >> 5);
+                    const float v = Mk[o] + Mb[o];
+                    Mk[o] = v;
+                    mg[e] = v;
+                }
             }
             __syncthreads();
             double e = 0.0;
@@ -369,6 +398,9 @@ __device__ double energy_sweep32(const W32 &C, int pc, double *On, double *Ou, b
     }
     for (int k = L - 1; k >= 1; --k) {
         float *cen = C.blk(pc), *nxt = C.blk(pj);
+        frag16 fv, fg;  // trial operands of stages 2 and 3
+        load_frag(fv, C.tn(k, w >> 1));
+        load_frag(fg, C.tt(k - 1, w >> 1));
         g2l(C.ph(k - 1, 0), nxt);
         g2l(C.ph(k - 1, 1), nxt + BLK);
         g2l(C.Lg + (size_t)(k - 1) * GB, C.blk(B_L));
@@ -390,7 +422,7 @@ __device__ double energy_sweep32(const W32 &C, int pc, double *On, double *Ou, b
         // stage 2: V(t',t) = T_k[t'] W[t] (4 waves); Y[s] = L_{k-1} phi'_{k-1}[s] (waves 0,1)
         {
             v16f acc = zero16();
-            mma<false, false>(acc, C.tn(k, w >> 1), 32, C.blk(B_W) + (w & 1) * BLK, LD);
+            mma_frag(acc, fv, C.blk(B_W) + (w & 1) * BLK, LD);
             store(acc, C.blk(B_V) + w * BLK, LD);
             if (w < 2) {
                 v16f y = zero16();
@@ -408,7 +440,7 @@ __device__ double energy_sweep32(const W32 &C, int pc, double *On, double *Ou, b
             float *R = C.blk(B_R);
             for (int e = threadIdx.x; e < BLK; e += MQE_NT) R[e] = V[e] + V[3 * BLK + e];
             v16f g = zero16();
-            mma<false, false>(g, C.tt(k - 1, w >> 1), 32, C.blk(B_Y) + (w & 1) * BLK, LD);
+            mma_frag(g, fg, C.blk(B_Y) + (w & 1) * BLK, LD);
             const int sp = w >> 1, ss = w & 1;
             for (int q = 0; q < 4; ++q) {
                 const double d = dot_tile(g, V + q * BLK, LD);
@@ -449,7 +481,7 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
     const int L = S.L;
     __shared__ int wv[MQE_MAXL + 1];
     __shared__ double red[MQE_NT + 16 * MQE_WAVES];
-    __shared__ double M[16], comb[16], Ux[4], Vx[4], scal[4];
+    __shared__ double M[16], scal[4];
     for (int k = threadIdx.x; k <= L; k += MQE_NT) wv[k] = P.wvd[(size_t)slot * (L + 1) + k];
     W32 C;
     C.S = &S;
@@ -502,20 +534,33 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
     }
 
     // ---------------- propagation (MPSQMC2.cpp:532-600), h = 0 ----------------
+    // The tensor that becomes the next coupling's left site is normalised
+    // lazily: its squared norm is known one stage after the coupling ends and
+    // the factor sc = 1/norm is folded into the next coupling's mixed elements
+    // and sampled operator.
     int pc = B_BUF0, pj = B_BUF1;
     g2l(C.ph(0, 0), C.blk(pc));
     g2l(C.ph(0, 1), C.blk(pc) + BLK);
     set_unit(C.blk(B_L));
     set_unit(C.blk(B_N));
-    __syncthreads();
+    double sc = 1.0, fstar = 0.0;
+    const int Ng = S.Ng;
     for (int c = 0; c < S.nC; ++c) {
         const int i = c, j = c + 1;
         float *cen = C.blk(pc), *nxt = C.blk(pj);
+        frag16 fg, fv;  // trial operands of stage B
+        load_frag(fg, C.tt(i, w >> 1));
+        load_frag(fv, C.tn(j, w >> 1));
         g2l(C.ph(j, 0), nxt);
         g2l(C.ph(j, 1), nxt + BLK);
         g2l(C.Rg + (size_t)(j + 1) * GB, C.blk(B_R));
         g2l(C.Rg + (size_t)(L + 2 + j) * GB, C.blk(B_LM));  // right norm environment M_{j+1}
         __syncthreads();
+        if (c > 0) {  // squared norm of the previous coupling's result (stage G)
+            const double nrm = sqrt(scal[1]);
+            sc = 1.0 / nrm;
+            O = 4.0 * fstar / nrm;
+        }
         // stage A: Y[s] = L_i phi_i[s] (waves 0,1); W[t] = R_{j+1} phi_j[t]^T (waves 2,3)
         {
             v16f acc = zero16();
@@ -531,23 +576,28 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
         // stage B: G(s',s) = T_i[s']^T Y[s] kept in registers; V(t',t) = T_j[t'] W[t] to LDS
         const int sp = w >> 1, ss = w & 1;
         v16f g = zero16();
-        mma<false, false>(g, C.tt(i, sp), 32, C.blk(B_Y) + ss * BLK, LD);
+        mma_frag(g, fg, C.blk(B_Y) + ss * BLK, LD);
         {
             v16f v = zero16();
-            mma<false, false>(v, C.tn(j, w >> 1), 32, C.blk(B_W) + (w & 1) * BLK, LD);
+            mma_frag(v, fv, C.blk(B_W) + (w & 1) * BLK, LD);
             store(v, C.blk(B_V) + w * BLK, LD);
         }
         __syncthreads();
-        // stage C: the 16 two-site mixed matrix elements
+        // stage C: the 16 two-site mixed matrix elements; G(s',s) parked in LDS
+        // (B_Y, B_W are free after stage B) for L_{i+1}
         for (int q = 0; q < 4; ++q) {
             const double d = dot_tile(g, C.blk(B_V) + q * BLK, LD);
             if (lane == 0) M[(sp + 2 * (q >> 1)) * 4 + (ss + 2 * (q & 1))] = d;
         }
+        store(g, (w < 2) ? C.blk(B_Y) + w * BLK : C.blk(B_W) + (w - 2) * BLK, LD);
         __syncthreads();
-        // stage D: combos, PDF, CDF sample (MPSQMC2.cpp:540-588)
+        // stage D: combos, PDF, CDF sample (MPSQMC2.cpp:540-588).  Every wave
+        // computes the same result, so no workgroup barrier is needed: lane
+        // q < 16 forms combo q, lane l owns grid points 4l .. 4l+3.
         const double *tS = S.tS + 4 * c, *tU = S.tU + 16 * c, *tVT = S.tVT + 16 * c;
-        if (threadIdx.x < 16) {
-            const int a = threadIdx.x & 3, bb = threadIdx.x >> 2;
+        double cq = 0.0;
+        if (lane < 16) {
+            const int a = lane & 3, bb = lane >> 2;
             double sacc = 0.0;
             for (int xsp = 0; xsp < 2; ++xsp)
                 for (int xss = 0; xss < 2; ++xss) {
@@ -557,71 +607,82 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
                         for (int t = 0; t < 2; ++t)
                             sacc += ua * tVT[bb + 4 * (tp + 2 * t)] * M[(xsp + 2 * tp) * 4 + (xss + 2 * t)];
                 }
-            comb[threadIdx.x] = sqrt(tS[a] * tS[bb]) * sacc;
+            cq = sqrt(tS[a] * tS[bb]) * (sacc * sc);
         }
-        __syncthreads();
-        double *pdf = (double *)C.blk(B_W), *cum = pdf + MAXG_T32;
-        const int Ng = S.Ng;
+        double cb[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) cb[q] = readlane_d(cq, q);
         const double pref = 4.0 / (O * Ng);
-        double part = 0.0;
-        for (int gq = threadIdx.x; gq < Ng; gq += MQE_NT) {
-            const double *x = S.grid + 4 * gq;
-            double f = 0.0;
-            for (int l = 0; l < 4; ++l)
-                for (int r = 0; r < 4; ++r) f += comb[l + 4 * r] * x[l] * x[r];
-            double v = f > 0.0 ? f : 0.0;
-            v *= pref;
-            pdf[gq] = v;
-            part += v;
+        double pv[4], loc = 0.0;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int gq = 4 * lane + t;
+            double v = 0.0;
+            if (gq < Ng) {
+                const double *x = S.grid + 4 * gq;
+                double f = 0.0;
+#pragma unroll
+                for (int l = 0; l < 4; ++l)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) f += cb[l + 4 * r] * x[l] * x[r];
+                v = (f > 0.0 ? f : 0.0) * pref;
+            }
+            pv[t] = v;
+            loc += v;
         }
-        const double pdfnorm = block_sum(part, red);
+        const double pdfnorm = wave_sum(loc);
         const double inv = 1.0 / pdfnorm;
-        for (int gq = threadIdx.x; gq < Ng; gq += MQE_NT) pdf[gq] *= inv;
-        __syncthreads();
-        block_inclusive_scan(Ng, pdf, cum, red);
+        double ln = 0.0;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            pv[t] *= inv;
+            ln += pv[t];
+        }
         const double u = P.u ? P.u[(size_t)b * S.nC + c]
                              : philox_uniform(P.seed, (uint32_t)b, (uint32_t)c, (uint32_t)P.step, 0u);
-        double cnt = 0.0;
-        for (int gq = threadIdx.x; gq < Ng - 1; gq += MQE_NT) cnt += (cum[gq] <= u) ? 1.0 : 0.0;
-        cnt = block_sum(cnt, red);
-        const int gs = (int)cnt;
+        double run = wave_exclusive_scan(ln), cnt = 0.0;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            run += pv[t];
+            if (4 * lane + t < Ng - 1 && run <= u) cnt += 1.0;
+        }
+        const int gs = (int)wave_sum(cnt);
         wt *= pdfnorm;
         if (threadIdx.x == 0) {
             if (P.idx) P.idx[(size_t)b * S.nC + c] = gs;
             if (!(pdfnorm > 0.0) || !isfinite(pdfnorm)) atomicOr(P.status, 1);
-            const double *x = S.grid + 4 * gs;
-            double f = 0.0;
-            for (int l = 0; l < 4; ++l)
-                for (int r = 0; r < 4; ++r) f += comb[l + 4 * r] * x[l] * x[r];
-            scal[0] = f;
-            for (int q = 0; q < 4; ++q) { Ux[q] = 0.0; Vx[q] = 0.0; }
-            for (int kk = 0; kk < 4; ++kk) {
-                const double pf = sqrt(tS[kk] * 4.0) * x[kk];
-                if (pf == 0.0) continue;
-                for (int up = 0; up < 2; ++up)
-                    for (int dn = 0; dn < 2; ++dn) {
-                        Ux[up + 2 * dn] += pf * tU[up + 2 * (dn + 2 * kk)];
-                        Vx[up + 2 * dn] += pf * tVT[kk + 4 * (up + 2 * dn)];
-                    }
-            }
         }
-        __syncthreads();
-        const double fstar = scal[0];
-        // stage E: apply the sampled operators; each wave scales its G tile for L_{i+1}
+        const double *xs = S.grid + 4 * gs;
+        fstar = 0.0;
+#pragma unroll
+        for (int l = 0; l < 4; ++l)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) fstar += cb[l + 4 * r] * xs[l] * xs[r];
+        double Ux[4] = {0.0, 0.0, 0.0, 0.0}, Vx[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            const double pf = sqrt(tS[kk] * 4.0) * xs[kk];
+#pragma unroll
+            for (int up = 0; up < 2; ++up)
+#pragma unroll
+                for (int dn = 0; dn < 2; ++dn) {
+                    Ux[up + 2 * dn] += pf * tU[up + 2 * (dn + 2 * kk)];
+                    Vx[up + 2 * dn] += pf * tVT[kk + 4 * (up + 2 * dn)];
+                }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) Ux[q] *= sc;  // left site carries the deferred normalisation
+        // stage E: apply the sampled operators
         apply2(cen, Ux);
         apply2(nxt, Vx);
-        {
-            const float ucoef = (float)Ux[sp + 2 * ss];
-            v16f gs2 = g * ucoef;
-            float *dst = (w < 2) ? C.blk(B_Y) + w * BLK : C.blk(B_W) + (w - 2) * BLK;
-            store(gs2, dst, LD);
-        }
         __syncthreads();
-        // stage F: L_{i+1} = sum of the four scaled tiles; X[s] = N_i phi'_i[s]; phi'_i is final
+        // stage F: L_{i+1} = sum_{s',s} Ux(s',s) G(s',s); X[s] = N_i phi'_i[s]; phi'_i is final
         {
             float *Ln = C.blk(B_L);
             const float *y0 = C.blk(B_Y), *w0 = C.blk(B_W);
-            for (int e = threadIdx.x; e < BLK; e += MQE_NT) Ln[e] = y0[e] + y0[BLK + e] + w0[e] + w0[BLK + e];
+            const float u00 = (float)Ux[0], u01 = (float)Ux[2], u10 = (float)Ux[1], u11 = (float)Ux[3];
+            for (int e = threadIdx.x; e < BLK; e += MQE_NT)
+                Ln[e] = u00 * y0[e] + u01 * y0[BLK + e] + u10 * w0[e] + u11 * w0[BLK + e];
             l2g(cen, C.ph(i, 0));
             l2g(cen + BLK, C.ph(i, 1));
             if (w < 2) {
@@ -629,36 +690,40 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
                 mma<false, false>(x, C.blk(B_N), LD, cen + w * BLK, LD);
                 store(x, C.blk(B_V) + w * BLK, LD);
             } else {
-                // Z1[t] = phi'_j[t] M_{j+1}
+                // Z[t] = phi'_j[t] M_{j+1}
                 v16f z1 = zero16();
                 mma<false, false>(z1, nxt + (w - 2) * BLK, LD, C.blk(B_LM), LD);
                 store(z1, C.blk(B_V) + w * BLK, LD);
             }
         }
         __syncthreads();
-        // stage G: N_{i+1} = sum_s phi'_i[s]^T X[s] (wave 0); L_{i+1} to global
+        // stage G: N_{i+1} = sum_s phi'_i[s]^T X[s] (wave 0);
+        // K^T = sum_t phi'_j[t] Z[t]^T (wave 1), squared norm = <N_{i+1}, K^T> = tr(N_{i+1} K)
+        v16f kt = zero16();
         if (w == 0) {
             v16f a = zero16();
             mma<true, false>(a, cen, LD, C.blk(B_V), LD);
             mma<true, false>(a, cen + BLK, LD, C.blk(B_V) + BLK, LD);
             store(a, C.blk(B_N), LD);
+        } else if (w == 1) {
+            mma<false, true>(kt, nxt, LD, C.blk(B_V) + 2 * BLK, LD);
+            mma<false, true>(kt, nxt + BLK, LD, C.blk(B_V) + 3 * BLK, LD);
         }
         l2g(C.blk(B_L), C.Lg + (size_t)(i + 1) * GB);
         __syncthreads();
-        // stage H: squared norm = sum_t <phi'_j[t], N_{i+1} phi'_j[t] M_{j+1}>
-        if (w < 2) {
-            v16f z = zero16();
-            mma<false, false>(z, C.blk(B_N), LD, C.blk(B_V) + (2 + w) * BLK, LD);
-            const double d = dot_tile(z, nxt + w * BLK, LD);
-            if (lane == 0) scal[1 + w] = d;
+        if (w == 1) {  // read before the next stage G rewrites B_N (after two more barriers)
+            const double d = dot_tile(kt, C.blk(B_N), LD);
+            if (lane == 0) scal[1] = d;
         }
-        __syncthreads();
-        const double nrm = sqrt(scal[1] + scal[2]);
-        scale_blocks(nxt, 2, 1.0 / nrm);
-        O = 4.0 * fstar / nrm;
         int t = pc; pc = pj; pj = t;
-        __syncthreads();
     }
+    __syncthreads();
+    {
+        const double nrm = sqrt(scal[1]);
+        O = 4.0 * fstar / nrm;
+        scale_blocks(C.blk(pc), 2, 1.0 / nrm);
+    }
+    __syncthreads();
     // ---------------- local energy (Walker.cpp:72-77) ----------------
     double On, Ou;
     const double e = energy_sweep32(C, pc, &On, &Ou, (flags & F_CANON) != 0);
