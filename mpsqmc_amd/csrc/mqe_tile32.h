@@ -109,6 +109,26 @@ __device__ __forceinline__ void g2l(const float *g, float *s) {
 __device__ __forceinline__ void l2g(const float *s, float *g) {
     for (int e = threadIdx.x; e < GB; e += MQE_NT) g[e] = s[(e & 31) + LD * (e >> 5)];
 }
+// NB global padded blocks -> registers (element tid + 256 n of block b in
+// r[4 b + n]) and registers -> LDS blocks; used to fetch the next coupling's
+// operands while the current one computes.
+template <int NB>
+__device__ __forceinline__ void pre_load(float (&r)[16], const float *const (&g)[NB]) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int n = 0; n < 4; ++n) r[4 * b + n] = g[b][threadIdx.x + MQE_NT * n];
+}
+template <int NB>
+__device__ __forceinline__ void pre_store(const float (&r)[16], float *const (&s)[NB]) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int n = 0; n < 4; ++n) {
+            const int e = threadIdx.x + MQE_NT * n;
+            s[b][(e & 31) + LD * (e >> 5)] = r[4 * b + n];
+        }
+}
 __device__ __forceinline__ void set_unit(float *s) {  // 1 at (0,0), 0 elsewhere
     for (int e = threadIdx.x; e < BLK; e += MQE_NT) s[e] = (e == 0) ? 1.f : 0.f;
 }
@@ -315,15 +335,25 @@ __device__ double energy_sweep32(const W32 &C, int pc, double *On, double *Ou, b
         // The last site's propagated tensor is only in LDS: store it.
         l2g(C.blk(pc), C.ph(L - 1, 0));
         l2g(C.blk(pc) + BLK, C.ph(L - 1, 1));
+        float pre[16];  // phi_{k-1} and L_{k-1}, fetched one site ahead
+        {
+            const float *const g[3] = {C.ph(L - 2, 0), C.ph(L - 2, 1), C.Lg + (size_t)(L - 2) * GB};
+            pre_load<3>(pre, g);
+        }
         for (int k = L - 1; k >= 1; --k) {
             float *cen = C.blk(pc), *nxt = C.blk(pj);
             frag16 fv, fg;  // trial operands of stages 2 and 4
             load_frag(fv, C.tn(k, w >> 1));
             load_frag(fg, C.tt(k - 1, w >> 1));
-            g2l(C.ph(k - 1, 0), nxt);
-            g2l(C.ph(k - 1, 1), nxt + BLK);
-            g2l(C.Lg + (size_t)(k - 1) * GB, C.blk(B_L));
+            {
+                float *const d[3] = {nxt, nxt + BLK, C.blk(B_L)};
+                pre_store<3>(pre, d);
+            }
             __syncthreads();
+            if (k >= 2) {
+                const float *const g[3] = {C.ph(k - 2, 0), C.ph(k - 2, 1), C.Lg + (size_t)(k - 2) * GB};
+                pre_load<3>(pre, g);
+            }
             {  // stage 1: W[t] = R_{k+1} phi_k[t]^T (waves 0,1); X[t] = phi_k[t] M_{k+1} (waves 2,3)
                 v16f acc = zero16();
                 if (w < 2) {
@@ -545,17 +575,29 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
     set_unit(C.blk(B_N));
     double sc = 1.0, fstar = 0.0;
     const int Ng = S.Ng;
+    // operands of coupling c, fetched one coupling ahead: phi_{c+1}, R_{c+2}
+    // and the right norm environment M_{c+2}
+    float pre[16];
+    {
+        const float *const g[4] = {C.ph(1, 0), C.ph(1, 1), C.Rg + (size_t)2 * GB, C.Rg + (size_t)(L + 3) * GB};
+        pre_load<4>(pre, g);
+    }
     for (int c = 0; c < S.nC; ++c) {
         const int i = c, j = c + 1;
         float *cen = C.blk(pc), *nxt = C.blk(pj);
         frag16 fg, fv;  // trial operands of stage B
         load_frag(fg, C.tt(i, w >> 1));
         load_frag(fv, C.tn(j, w >> 1));
-        g2l(C.ph(j, 0), nxt);
-        g2l(C.ph(j, 1), nxt + BLK);
-        g2l(C.Rg + (size_t)(j + 1) * GB, C.blk(B_R));
-        g2l(C.Rg + (size_t)(L + 2 + j) * GB, C.blk(B_LM));  // right norm environment M_{j+1}
+        {
+            float *const d[4] = {nxt, nxt + BLK, C.blk(B_R), C.blk(B_LM)};
+            pre_store<4>(pre, d);
+        }
         __syncthreads();
+        if (c + 1 < S.nC) {  // fetch the next coupling's operands
+            const float *const g[4] = {C.ph(j + 1, 0), C.ph(j + 1, 1), C.Rg + (size_t)(j + 2) * GB,
+                                       C.Rg + (size_t)(L + 3 + j) * GB};
+            pre_load<4>(pre, g);
+        }
         if (c > 0) {  // squared norm of the previous coupling's result (stage G)
             const double nrm = sqrt(scal[1]);
             sc = 1.0 / nrm;
