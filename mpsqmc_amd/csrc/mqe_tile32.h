@@ -102,6 +102,29 @@ __device__ __forceinline__ double dot_tile(const v16f &acc, const float *C, int 
     return wave_sum(s);
 }
 
+// The four dots <acc, B_q>, B_q = base + q BLK (q = 0..3), FP64 accumulation;
+// lane l receives dot l >> 4 (a reduce-scatter over the lane halves, then a
+// DPP sum over each 16-lane row).
+__device__ __forceinline__ double dot4_tile(const v16f &acc, const float *base, int ldc) {
+    const int j = col_of(), lane = threadIdx.x & 63;
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) s[q] += (double)acc[e] * (double)base[q * BLK + row_of(e) + ldc * j];
+    const bool hi = lane >= 32, h16 = (lane & 16) != 0;
+    double a0 = hi ? s[2] : s[0], a1 = hi ? s[3] : s[1];
+    const double b0 = hi ? s[0] : s[2], b1 = hi ? s[1] : s[3];
+    a0 += __shfl_xor(b0, 32, 64);
+    a1 += __shfl_xor(b1, 32, 64);
+    double c0 = h16 ? a1 : a0;
+    const double c1 = h16 ? a0 : a1;
+    c0 += __shfl_xor(c1, 16, 64);
+    c0 = group8_sum(c0);
+    c0 += dpp_d<0x140>(c0);
+    return c0;
+}
+
 // global padded block -> LDS block (whole workgroup)
 __device__ __forceinline__ void g2l(const float *g, float *s) {
     for (int e = threadIdx.x; e < GB; e += MQE_NT) s[(e & 31) + LD * (e >> 5)] = g[e];
@@ -391,9 +414,10 @@ __device__ double energy_sweep32(const W32 &C, int pc, double *On, double *Ou, b
                 v16f g = zero16();
                 mma_frag(g, fg, C.blk(B_W) + (w & 1) * BLK, LD);
                 const int sp = w >> 1, ss = w & 1;
-                for (int q = 0; q < 4; ++q) {
-                    const double d = dot_tile(g, V + q * BLK, LD);
-                    if (lane == 0) Mq[(sp + 2 * (q >> 1)) * 4 + (ss + 2 * (q & 1))] = d;
+                {
+                    const double d = dot4_tile(g, V, LD);
+                    const int q = lane >> 4;
+                    if ((lane & 15) == 0) Mq[(sp + 2 * (q >> 1)) * 4 + (ss + 2 * (q & 1))] = d;
                 }
                 l2g(C.blk(B_R), C.Rg + (size_t)k * GB);
                 float *Mk = C.blk(B_LM);
@@ -472,9 +496,10 @@ __device__ double energy_sweep32(const W32 &C, int pc, double *On, double *Ou, b
             v16f g = zero16();
             mma_frag(g, fg, C.blk(B_Y) + (w & 1) * BLK, LD);
             const int sp = w >> 1, ss = w & 1;
-            for (int q = 0; q < 4; ++q) {
-                const double d = dot_tile(g, V + q * BLK, LD);
-                if (lane == 0) Mq[(sp + 2 * (q >> 1)) * 4 + (ss + 2 * (q & 1))] = d;
+            {
+                const double d = dot4_tile(g, V, LD);
+                const int q = lane >> 4;
+                if ((lane & 15) == 0) Mq[(sp + 2 * (q >> 1)) * 4 + (ss + 2 * (q & 1))] = d;
             }
         }
         __syncthreads();
@@ -510,8 +535,10 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
     const int slot = P.order[b];
     const int L = S.L;
     __shared__ int wv[MQE_MAXL + 1];
-    __shared__ double red[MQE_NT + 16 * MQE_WAVES];
+    __shared__ double red[2 * MQE_WAVES];
     __shared__ double M[16], scal[4];
+    __shared__ double grid_s[4 * MAXG_T32];  // the Marsaglia grid
+    __shared__ double tab_s[2][36];          // Trotter tables S, U, VT of the current and the next coupling
     for (int k = threadIdx.x; k <= L; k += MQE_NT) wv[k] = P.wvd[(size_t)slot * (L + 1) + k];
     W32 C;
     C.S = &S;
@@ -582,6 +609,13 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
         const float *const g[4] = {C.ph(1, 0), C.ph(1, 1), C.Rg + (size_t)2 * GB, C.Rg + (size_t)(L + 3) * GB};
         pre_load<4>(pre, g);
     }
+    for (int e = threadIdx.x; e < 4 * Ng; e += MQE_NT) grid_s[e] = S.grid[e];
+    // tables of coupling c: tab[0..3] = S, tab[4..19] = U, tab[20..35] = VT
+    auto table_entry = [&](int c, int t) {
+        return t < 4 ? S.tS[4 * c + t] : (t < 20 ? S.tU[16 * c + t - 4] : S.tVT[16 * c + t - 20]);
+    };
+    if (threadIdx.x < 36) tab_s[0][threadIdx.x] = table_entry(0, threadIdx.x);
+    double tnext = (threadIdx.x < 36 && S.nC > 1) ? table_entry(1, threadIdx.x) : 0.0;
     for (int c = 0; c < S.nC; ++c) {
         const int i = c, j = c + 1;
         float *cen = C.blk(pc), *nxt = C.blk(pj);
@@ -592,11 +626,13 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
             float *const d[4] = {nxt, nxt + BLK, C.blk(B_R), C.blk(B_LM)};
             pre_store<4>(pre, d);
         }
+        if (c > 0 && threadIdx.x < 36) tab_s[c & 1][threadIdx.x] = tnext;
         __syncthreads();
         if (c + 1 < S.nC) {  // fetch the next coupling's operands
             const float *const g[4] = {C.ph(j + 1, 0), C.ph(j + 1, 1), C.Rg + (size_t)(j + 2) * GB,
                                        C.Rg + (size_t)(L + 3 + j) * GB};
             pre_load<4>(pre, g);
+            if (c > 0 && threadIdx.x < 36) tnext = table_entry(c + 1, threadIdx.x);
         }
         if (c > 0) {  // squared norm of the previous coupling's result (stage G)
             const double nrm = sqrt(scal[1]);
@@ -627,16 +663,17 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
         __syncthreads();
         // stage C: the 16 two-site mixed matrix elements; G(s',s) parked in LDS
         // (B_Y, B_W are free after stage B) for L_{i+1}
-        for (int q = 0; q < 4; ++q) {
-            const double d = dot_tile(g, C.blk(B_V) + q * BLK, LD);
-            if (lane == 0) M[(sp + 2 * (q >> 1)) * 4 + (ss + 2 * (q & 1))] = d;
+        {
+            const double d = dot4_tile(g, C.blk(B_V), LD);
+            const int q = lane >> 4;
+            if ((lane & 15) == 0) M[(sp + 2 * (q >> 1)) * 4 + (ss + 2 * (q & 1))] = d;
         }
         store(g, (w < 2) ? C.blk(B_Y) + w * BLK : C.blk(B_W) + (w - 2) * BLK, LD);
         __syncthreads();
         // stage D: combos, PDF, CDF sample (MPSQMC2.cpp:540-588).  Every wave
         // computes the same result, so no workgroup barrier is needed: lane
         // q < 16 forms combo q, lane l owns grid points 4l .. 4l+3.
-        const double *tS = S.tS + 4 * c, *tU = S.tU + 16 * c, *tVT = S.tVT + 16 * c;
+        const double *tS = tab_s[c & 1], *tU = tS + 4, *tVT = tS + 20;
         double cq = 0.0;
         if (lane < 16) {
             const int a = lane & 3, bb = lane >> 2;
@@ -661,7 +698,7 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
             const int gq = 4 * lane + t;
             double v = 0.0;
             if (gq < Ng) {
-                const double *x = S.grid + 4 * gq;
+                const double *x = grid_s + 4 * gq;
                 double f = 0.0;
 #pragma unroll
                 for (int l = 0; l < 4; ++l)
@@ -694,7 +731,7 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
             if (P.idx) P.idx[(size_t)b * S.nC + c] = gs;
             if (!(pdfnorm > 0.0) || !isfinite(pdfnorm)) atomicOr(P.status, 1);
         }
-        const double *xs = S.grid + 4 * gs;
+        const double *xs = grid_s + 4 * gs;
         fstar = 0.0;
 #pragma unroll
         for (int l = 0; l < 4; ++l)
