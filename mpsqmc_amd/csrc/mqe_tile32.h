@@ -84,6 +84,65 @@ __device__ __forceinline__ void mma_frag(v16f &acc, const frag16 &a, const float
     }
 }
 
+// MFMA result (accumulator layout: slot q of lane (j, h) holds row
+// 8(q/4) + 4h + q%4 of column j) -> B-operand fragment of the same matrix
+// (b[q] = row 2q + h of column j): half of the rows move between lanes l and
+// l ^ 32.
+__device__ __forceinline__ void acc_to_bfrag(const v16f &acc, frag16 &b) {
+    const int h = (threadIdx.x & 63) >> 5;
+#pragma unroll
+    for (int g = 0; g < 4; ++g)
+#pragma unroll
+        for (int o = 0; o < 2; ++o) {
+            const int base = 4 * g + 2 * o, q = 4 * g + o;
+            const float keep = h ? acc[base + 1] : acc[base];
+            const float send = h ? acc[base] : acc[base + 1];
+            const float recv = __shfl_xor(send, 32, 64);
+            b[q] = h ? recv : keep;
+            b[q | 2] = h ? keep : recv;
+        }
+}
+
+// acc += A B with both operands as register fragments
+__device__ __forceinline__ void mma_rr(v16f &acc, const frag16 &a, const frag16 &b) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[q], b[q], acc, 0, 0, 0);
+}
+
+// acc += op(A) B, A in LDS (TA: stored transposed), B a register fragment
+template <bool TA>
+__device__ __forceinline__ void mma_lr(v16f &acc, const float *A, int lda, const frag16 &b) {
+    const int l = threadIdx.x & 63, r = l & 31, h = l >> 5;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int k = 2 * q + h;
+        const float a = TA ? A[k + lda * r] : A[r + lda * k];
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b[q], acc, 0, 0, 0);
+    }
+}
+
+// Fragments of X = c0 X0 + c1 X1 (two LDS blocks, leading dim LD) formed on
+// the fly.  A fragment: a[q] = X(r, 2q + h); B fragment: b[q] = X(2q + h, j);
+// T: the blocks hold X transposed.
+template <bool T>
+__device__ __forceinline__ void afrag_mix(frag16 &a, const float *X0, const float *X1, float c0, float c1) {
+    const int l = threadIdx.x & 63, r = l & 31, h = l >> 5;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int k = 2 * q + h, o = T ? k + LD * r : r + LD * k;
+        a[q] = c0 * X0[o] + c1 * X1[o];
+    }
+}
+template <bool T>
+__device__ __forceinline__ void bfrag_mix(frag16 &b, const float *X0, const float *X1, float c0, float c1) {
+    const int l = threadIdx.x & 63, j = l & 31, h = l >> 5;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int k = 2 * q + h, o = T ? j + LD * k : k + LD * j;
+        b[q] = c0 * X0[o] + c1 * X1[o];
+    }
+}
+
 // element (i, j) held by this lane in accumulator slot q
 __device__ __forceinline__ int row_of(int q) { return 8 * (q >> 2) + 4 * ((threadIdx.x & 63) >> 5) + (q & 3); }
 __device__ __forceinline__ int col_of() { return threadIdx.x & 31; }
@@ -94,24 +153,45 @@ __device__ __forceinline__ void store(const v16f &acc, float *C, int ldc) {
     for (int q = 0; q < 16; ++q) C[row_of(q) + ldc * j] = acc[q];
 }
 
-__device__ __forceinline__ double dot_tile(const v16f &acc, const float *C, int ldc) {
+// FP32 partial of <acc, B> over this lane's 16 elements
+__device__ __forceinline__ float dot_partial(const v16f &acc, const float *B, int ldb) {
     const int j = col_of();
-    double s = 0.0;
+    float f = 0.f;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) s += (double)acc[q] * (double)C[row_of(q) + ldc * j];
-    return wave_sum(s);
+    for (int e = 0; e < 16; ++e) f += acc[e] * B[row_of(e) + ldb * j];
+    return f;
 }
 
-// The four dots <acc, B_q>, B_q = base + q BLK (q = 0..3), FP64 accumulation;
-// lane l receives dot l >> 4 (a reduce-scatter over the lane halves, then a
-// DPP sum over each 16-lane row).
+// four wave-wide sums: lane l receives the sum of s_{l >> 4}
+__device__ __forceinline__ double reduce4(double s0, double s1, double s2, double s3) {
+    const int lane = threadIdx.x & 63;
+    const bool hi = lane >= 32, h16 = (lane & 16) != 0;
+    double a0 = hi ? s2 : s0, a1 = hi ? s3 : s1;
+    const double b0 = hi ? s0 : s2, b1 = hi ? s1 : s3;
+    a0 += __shfl_xor(b0, 32, 64);
+    a1 += __shfl_xor(b1, 32, 64);
+    double c0 = h16 ? a1 : a0;
+    const double c1 = h16 ? a0 : a1;
+    c0 += __shfl_xor(c1, 16, 64);
+    c0 = group8_sum(c0);
+    c0 += dpp_d<0x140>(c0);
+    return c0;
+}
+
+// The four dots <acc, B_q>, B_q = base + q BLK (q = 0..3): each lane sums its
+// 16 products in FP32 (the operands are FP32 MFMA results), the cross-lane
+// sum is FP64.  Lane l receives dot l >> 4 (a reduce-scatter over the lane
+// halves, then a DPP sum over each 16-lane row).
 __device__ __forceinline__ double dot4_tile(const v16f &acc, const float *base, int ldc) {
     const int j = col_of(), lane = threadIdx.x & 63;
-    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    double s[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+    for (int q = 0; q < 4; ++q) {
+        float f = 0.f;
 #pragma unroll
-        for (int e = 0; e < 16; ++e) s[q] += (double)acc[e] * (double)base[q * BLK + row_of(e) + ldc * j];
+        for (int e = 0; e < 16; ++e) f += acc[e] * base[q * BLK + row_of(e) + ldc * j];
+        s[q] = (double)f;
+    }
     const bool hi = lane >= 32, h16 = (lane & 16) != 0;
     double a0 = hi ? s[2] : s[0], a1 = hi ? s[3] : s[1];
     const double b0 = hi ? s[0] : s[2], b1 = hi ? s[1] : s[3];
@@ -619,9 +699,15 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
     for (int c = 0; c < S.nC; ++c) {
         const int i = c, j = c + 1;
         float *cen = C.blk(pc), *nxt = C.blk(pj);
-        frag16 fg, fv;  // trial operands of stage B
-        load_frag(fg, C.tt(i, w >> 1));
-        load_frag(fv, C.tn(j, w >> 1));
+        // trial operands of stage AB: T_i[s']^T (waves 0,1) or T_j[t'] (waves 2,3), s', t' = 0, 1
+        frag16 f0, f1;
+        if (w < 2) {
+            load_frag(f0, C.tt(i, 0));
+            load_frag(f1, C.tt(i, 1));
+        } else {
+            load_frag(f0, C.tn(j, 0));
+            load_frag(f1, C.tn(j, 1));
+        }
         {
             float *const d[4] = {nxt, nxt + BLK, C.blk(B_R), C.blk(B_LM)};
             pre_store<4>(pre, d);
@@ -635,40 +721,49 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
             if (c > 0 && threadIdx.x < 36) tnext = table_entry(c + 1, threadIdx.x);
         }
         if (c > 0) {  // squared norm of the previous coupling's result (stage G)
-            const double nrm = sqrt(scal[1]);
+            const double nrm = sqrt(scal[0] + scal[1] + scal[2] + scal[3]);
             sc = 1.0 / nrm;
             O = 4.0 * fstar / nrm;
         }
-        // stage A: Y[s] = L_i phi_i[s] (waves 0,1); W[t] = R_{j+1} phi_j[t]^T (waves 2,3)
+        // stage AB, chained in registers:
+        //   waves 0,1 (s = w):     Y[s] = L_i phi_i[s],        G(s',s) = T_i[s']^T Y[s]
+        //   waves 2,3 (t = w - 2): W[t] = R_{j+1} phi_j[t]^T,  V(t',t) = T_j[t'] W[t]
+        // G(s',s) is stored at B_Y + (2s' + s), V(t',t) at B_V + (2t' + t).
+        v16f r0 = zero16(), r1 = zero16();
         {
-            v16f acc = zero16();
-            if (w < 2) {
-                mma<false, false>(acc, C.blk(B_L), LD, cen + w * BLK, LD);
-                store(acc, C.blk(B_Y) + w * BLK, LD);
-            } else {
-                mma<false, true>(acc, C.blk(B_R), LD, nxt + (w - 2) * BLK, LD);
-                store(acc, C.blk(B_W) + (w - 2) * BLK, LD);
-            }
+            v16f y = zero16();
+            if (w < 2)
+                mma<false, false>(y, C.blk(B_L), LD, cen + w * BLK, LD);
+            else
+                mma<false, true>(y, C.blk(B_R), LD, nxt + (w - 2) * BLK, LD);
+            frag16 yb;
+            acc_to_bfrag(y, yb);
+            mma_rr(r0, f0, yb);
+            mma_rr(r1, f1, yb);
+            float *base = (w < 2) ? C.blk(B_Y) : C.blk(B_V);
+            store(r0, base + (w & 1) * BLK, LD);
+            store(r1, base + (2 + (w & 1)) * BLK, LD);
         }
         __syncthreads();
-        // stage B: G(s',s) = T_i[s']^T Y[s] kept in registers; V(t',t) = T_j[t'] W[t] to LDS
-        const int sp = w >> 1, ss = w & 1;
-        v16f g = zero16();
-        mma_frag(g, fg, C.blk(B_Y) + ss * BLK, LD);
+        // stage C: the 16 two-site mixed matrix elements M[(s'+2t')4 + (s+2t)] = <G(s',s), V(t',t)>
+        //   waves 0,1: <G(0,s), V(t',t)> for all t', t;  waves 2,3: <V(t',t), G(1,s)> for all t', s
         {
-            v16f v = zero16();
-            mma_frag(v, fv, C.blk(B_W) + (w & 1) * BLK, LD);
-            store(v, C.blk(B_V) + w * BLK, LD);
-        }
-        __syncthreads();
-        // stage C: the 16 two-site mixed matrix elements; G(s',s) parked in LDS
-        // (B_Y, B_W are free after stage B) for L_{i+1}
-        {
-            const double d = dot4_tile(g, C.blk(B_V), LD);
+            double d;
+            int idx;
             const int q = lane >> 4;
-            if ((lane & 15) == 0) M[(sp + 2 * (q >> 1)) * 4 + (ss + 2 * (q & 1))] = d;
+            if (w < 2) {
+                const float *V = C.blk(B_V);
+                d = reduce4(dot_partial(r0, V, LD), dot_partial(r0, V + BLK, LD), dot_partial(r0, V + 2 * BLK, LD),
+                            dot_partial(r0, V + 3 * BLK, LD));
+                idx = (2 * (q >> 1)) * 4 + (w + 2 * (q & 1));
+            } else {
+                const float *G1 = C.blk(B_Y) + 2 * BLK;
+                d = reduce4(dot_partial(r0, G1, LD), dot_partial(r0, G1 + BLK, LD), dot_partial(r1, G1, LD),
+                            dot_partial(r1, G1 + BLK, LD));
+                idx = (1 + 2 * (q >> 1)) * 4 + ((q & 1) + 2 * (w - 2));
+            }
+            if ((lane & 15) == 0) M[idx] = d;
         }
-        store(g, (w < 2) ? C.blk(B_Y) + w * BLK : C.blk(B_W) + (w - 2) * BLK, LD);
         __syncthreads();
         // stage D: combos, PDF, CDF sample (MPSQMC2.cpp:540-588).  Every wave
         // computes the same result, so no workgroup barrier is needed: lane
@@ -751,54 +846,72 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) Ux[q] *= sc;  // left site carries the deferred normalisation
-        // stage E: apply the sampled operators
-        apply2(cen, Ux);
-        apply2(nxt, Vx);
-        __syncthreads();
-        // stage F: L_{i+1} = sum_{s',s} Ux(s',s) G(s',s); X[s] = N_i phi'_i[s]; phi'_i is final
+        // stage F: the sampled operators applied on the fly (phi' = Ux phi_i on the
+        // left site, Vx phi_j on the right one), chained in registers:
+        //   waves 0,1 (s):     X[s] = N_i phi'_i[s],       N^(s) = phi'_i[s]^T X[s];  phi'_i[s] to global
+        //   waves 2,3 (t):     Z[t] = M_{j+1} phi'_j[t]^T, K^(t)T = phi'_j[t] Z[t]
+        //   all:               L_{i+1} = sum_{s',s} Ux(s',s) G(s',s) (LDS and global)
         {
-            float *Ln = C.blk(B_L);
-            const float *y0 = C.blk(B_Y), *w0 = C.blk(B_W);
-            const float u00 = (float)Ux[0], u01 = (float)Ux[2], u10 = (float)Ux[1], u11 = (float)Ux[3];
-            for (int e = threadIdx.x; e < BLK; e += MQE_NT)
-                Ln[e] = u00 * y0[e] + u01 * y0[BLK + e] + u10 * w0[e] + u11 * w0[BLK + e];
-            l2g(cen, C.ph(i, 0));
-            l2g(cen + BLK, C.ph(i, 1));
-            if (w < 2) {
-                v16f x = zero16();
-                mma<false, false>(x, C.blk(B_N), LD, cen + w * BLK, LD);
-                store(x, C.blk(B_V) + w * BLK, LD);
-            } else {
-                // Z[t] = phi'_j[t] M_{j+1}
-                v16f z1 = zero16();
-                mma<false, false>(z1, nxt + (w - 2) * BLK, LD, C.blk(B_LM), LD);
-                store(z1, C.blk(B_V) + w * BLK, LD);
+            const float *G = C.blk(B_Y);
+            float *Ln = C.blk(B_L), *lg = C.Lg + (size_t)(i + 1) * GB;
+            const float u0 = (float)Ux[0], u1 = (float)Ux[1], u2 = (float)Ux[2], u3 = (float)Ux[3];
+            for (int e = threadIdx.x; e < GB; e += MQE_NT) {
+                const int o = (e & 31) + LD * (e >> 5);
+                const float v = u0 * G[o] + u2 * G[BLK + o] + u1 * G[2 * BLK + o] + u3 * G[3 * BLK + o];
+                Ln[o] = v;
+                lg[e] = v;
             }
-        }
-        __syncthreads();
-        // stage G: N_{i+1} = sum_s phi'_i[s]^T X[s] (wave 0);
-        // K^T = sum_t phi'_j[t] Z[t]^T (wave 1), squared norm = <N_{i+1}, K^T> = tr(N_{i+1} K)
-        v16f kt = zero16();
-        if (w == 0) {
             v16f a = zero16();
-            mma<true, false>(a, cen, LD, C.blk(B_V), LD);
-            mma<true, false>(a, cen + BLK, LD, C.blk(B_V) + BLK, LD);
-            store(a, C.blk(B_N), LD);
-        } else if (w == 1) {
-            mma<false, true>(kt, nxt, LD, C.blk(B_V) + 2 * BLK, LD);
-            mma<false, true>(kt, nxt + BLK, LD, C.blk(B_V) + 3 * BLK, LD);
+            if (w < 2) {
+                const float c0 = (float)Ux[w], c1 = (float)Ux[w + 2];
+                frag16 pb, xb, pa;
+                bfrag_mix<false>(pb, cen, cen + BLK, c0, c1);
+                v16f x = zero16();
+                mma_lr<false>(x, C.blk(B_N), LD, pb);
+                acc_to_bfrag(x, xb);
+                afrag_mix<true>(pa, cen, cen + BLK, c0, c1);
+                mma_rr(a, pa, xb);
+                float *pg = C.ph(i, w);
+#pragma unroll
+                for (int n = 0; n < 16; ++n) {
+                    const int e = lane + 64 * n, o = (e & 31) + LD * (e >> 5);
+                    pg[e] = c0 * cen[o] + c1 * cen[BLK + o];
+                }
+            } else {
+                const int t = w - 2;
+                const float c0 = (float)Vx[t], c1 = (float)Vx[t + 2];
+                frag16 pb, zb, pa;
+                bfrag_mix<true>(pb, nxt, nxt + BLK, c0, c1);
+                v16f z = zero16();
+                mma_lr<false>(z, C.blk(B_LM), LD, pb);
+                acc_to_bfrag(z, zb);
+                afrag_mix<false>(pa, nxt, nxt + BLK, c0, c1);
+                mma_rr(a, pa, zb);
+            }
+            store(a, C.blk(B_V) + w * BLK, LD);
         }
-        l2g(C.blk(B_L), C.Lg + (size_t)(i + 1) * GB);
         __syncthreads();
-        if (w == 1) {  // read before the next stage G rewrites B_N (after two more barriers)
-            const double d = dot_tile(kt, C.blk(B_N), LD);
-            if (lane == 0) scal[1] = d;
+        // stage G: N_{i+1} = N^(0) + N^(1); squared norm <N_{i+1}, K^(0)T + K^(1)T> = tr(N_{i+1} K),
+        // one partial per wave (read after the next barrier); phi'_j = Vx phi_j in place
+        {
+            const float *V = C.blk(B_V);
+            float *Nn = C.blk(B_N);
+            float part = 0.f;
+            for (int e = threadIdx.x; e < GB; e += MQE_NT) {
+                const int o = (e & 31) + LD * (e >> 5);
+                const float n = V[o] + V[BLK + o];
+                Nn[o] = n;
+                part += n * (V[2 * BLK + o] + V[3 * BLK + o]);
+            }
+            const double ws = wave_sum((double)part);
+            if (lane == 0) scal[w] = ws;
+            apply2(nxt, Vx);
         }
         int t = pc; pc = pj; pj = t;
     }
     __syncthreads();
     {
-        const double nrm = sqrt(scal[1]);
+        const double nrm = sqrt(scal[0] + scal[1] + scal[2] + scal[3]);
         O = 4.0 * fstar / nrm;
         scale_blocks(C.blk(pc), 2, 1.0 / nrm);
     }
