@@ -37,17 +37,48 @@ def log(msg):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=32)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--warmup", type=int, default=32)
     ap.add_argument("--walkers", type=int, default=2048)
     ap.add_argument("--L", type=int, default=32)
     ap.add_argument("--D", type=int, default=32)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
-    ap.add_argument("--cpu-walkers", type=int, default=128)
+    ap.add_argument("--cpu-walkers", type=int, default=512)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=20131015)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB ten times (reference for HBM counters)")
     return ap.parse_args()
+
+
+def calibrate_copy(mib):
+    """A plain copy kernel of known size: FETCH_SIZE / WRITE_SIZE of its dispatches
+    against mib MiB read and written give the counters' scale."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(10):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    log(f"bench: copied {mib} MiB ten times")
+
+
+TRAFFIC_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                               "walker32_kernel_chain_L32_D32_N2048_hbm.md")
+
+
+def measured_traffic(L, D, N):
+    """HBM bytes per walker-kernel launch from the committed counter summary
+    (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel reference),
+    for the workload it was measured on; None otherwise."""
+    if (L, D, N) != (32, 32, 2048) or not os.path.exists(TRAFFIC_SUMMARY):
+        return None
+    for line in open(TRAFFIC_SUMMARY):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
 
 
 def make_trial(L, D, seed):
@@ -80,6 +111,9 @@ def cpu_baseline(system, vd, data, walkers, threads):
 
 def main():
     args = parse()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               f"(walker sharding over GPUs is designed but not built)", file=sys.stderr)
@@ -98,6 +132,9 @@ def main():
     eng.run(1, args.warmup)
     eng.sync()
     log("bench: warmup done; timing")
+    if args.steps % 32:
+        log("bench: note: the tile path re-orthonormalises every 32 steps; time a multiple of 32 steps "
+            "for the amortised rate")
     eng.set_timing(True)
     first = args.warmup + 1
     t0 = time.perf_counter()
@@ -134,8 +171,8 @@ def main():
                                f"Marsaglia(4) grid, {args.precision} contractions",
                    "walkers": N, "L": L, "D": D},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
-                     "kernel": "walker_kernel", "avg_launch_ms": avg_launch_s * 1000.0,
+                     "frac": achieved / peak, "traffic": measured_traffic(L, D, N),
+                     "kernel": eng.kernel_name, "avg_launch_ms": avg_launch_s * 1000.0,
                      "flops_per_walker_step": F32_PER_WS + F64_PER_WS},
         "E_P_last": float(trace[-1, 2]),
         "population_last": int(trace[-1, 1]),

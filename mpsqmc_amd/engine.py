@@ -129,6 +129,11 @@ class Engine:
         check(lib().mqe_set_canonical_interval(self.h, steps))
 
     @property
+    def kernel_name(self):
+        """Name of the walker kernel this engine launches."""
+        return "walker32_kernel" if self.path == "tile32" else "walker_kernel"
+
+    @property
     def path(self):
         """'tile32' (FP32 MFMA path) or 'general'."""
         return "tile32" if lib().mqe_kernel_path(self.h) == 1 else "general"
