@@ -426,7 +426,7 @@ __device__ __forceinline__ void g_identity(float *g, int w) {
 __device__ double energy_sweep32(const W32 &C, int pc, double *On, double *Ou, bool canon) {
     const Sys &S = *C.S;
     const int L = S.L;
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     __shared__ double Mq[16];
     set_unit(C.blk(B_R));
     set_unit(C.blk(B_LM));  // M_L = 1 (light sweep)
@@ -630,7 +630,7 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
     C.sm = lds32;
     C.wv = wv;
     C.red = red;
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     double wt = P.w[slot], O = P.O[slot];
 
     if (flags & F_PREP) {
