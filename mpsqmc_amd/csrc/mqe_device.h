@@ -128,11 +128,65 @@ __device__ __forceinline__ void block_inclusive_scan(int n, const double *x, dou
 }
 
 // ---------------------------------------------------------------------------
+// FP32 GEMM on the matrix cores: C(m x n) (+)= op(A) op(B), whole workgroup.
+// The output is cut into 32 x 32 tiles, one wave per tile at a time; each tile
+// is a chain of v_mfma_f32_32x32x2f32 over k with operands read straight from
+// (L2-resident) memory, zero outside the matrices.  Lane l holds A(l % 32, kk)
+// and B(kk, l % 32) for kk = k0 + l / 32; accumulator slot q of lane l is
+// element (8(q/4) + 4(l/32) + q%4, l % 32) of the tile.
+// ---------------------------------------------------------------------------
+template <bool TA, bool TB>
+__device__ __forceinline__ void gemm_mfma(int m, int n, int k, const float *A, int lda, const float *B, int ldb,
+                                          float *C, int ldc, bool acc) {
+    typedef float v16 __attribute__((ext_vector_type(16)));
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int r = lane & 31, h = lane >> 5;
+    const int tm = (m + 31) >> 5, tn = (n + 31) >> 5;
+    for (int t = w; t < tm * tn; t += MQE_WAVES) {
+        const int r0 = (t % tm) * 32, c0 = (t / tm) * 32;
+        const int ar = r0 + r, bc = c0 + r;
+        const bool aok = ar < m, bok = bc < n;
+        v16 c;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) c[q] = 0.f;
+        for (int k0 = 0; k0 < k; k0 += 16) {
+            float a[8], b[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int kk = k0 + 2 * u + h;
+                const bool kok = kk < k;
+                a[u] = (aok && kok) ? (TA ? A[kk + (size_t)lda * ar] : A[ar + (size_t)lda * kk]) : 0.f;
+                b[u] = (bok && kok) ? (TB ? B[bc + (size_t)ldb * kk] : B[kk + (size_t)ldb * bc]) : 0.f;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) c = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u], b[u], c, 0, 0, 0);
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int rr = r0 + 8 * (q >> 2) + 4 * h + (q & 3);
+            if (rr < m && bok) {
+                float v = c[q];
+                if (acc) v += C[rr + (size_t)ldc * bc];
+                C[rr + (size_t)ldc * bc] = v;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // GEMM: C(m x n) (+)= op(A) op(B).  op(A) is m x k: A(r,q) = TA ? A[q + lda*r] : A[r + lda*q].
+// FP32 products of useful size run on the matrix cores, the rest (and all of
+// FP64) on the vector units.
 // ---------------------------------------------------------------------------
 template <bool TA, bool TB, typename T>
 __device__ __forceinline__ void gemm(int m, int n, int k, const T *A, int lda, const T *B, int ldb, T *C, int ldc,
                                      bool acc) {
+    if constexpr (sizeof(T) == sizeof(float)) {
+        if (m >= 8 && n >= 8 && k >= 4) {
+            gemm_mfma<TA, TB>(m, n, k, (const float *)A, lda, (const float *)B, ldb, (float *)C, ldc, acc);
+            return;
+        }
+    }
     const int mn = m * n;
     for (int e = threadIdx.x; e < mn; e += MQE_NT) {
         const int r = e % m, c = e / m;

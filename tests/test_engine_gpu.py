@@ -152,6 +152,18 @@ def test_parity_chain_fp32():
     _run_parity(systems.chain(8), 4, 12, 15, steps=2, precision="fp32", tol=2e-4, check_states=False)
 
 
+def test_parity_snake_fp32_general():
+    # long-range couplings take the general path; bonds of 16 run its products on the matrix cores
+    _run_parity(systems.snake_lattice(3, 3), 16, 8, 16, steps=2, precision="fp32", tol=2e-4, check_states=False)
+
+
+def test_parity_chain_field_fp32_general():
+    # With a field the FP32 general path shows per-step weight deviations up to ~1e-3 at
+    # bonds of 16 (measured: 8.9e-4 at h = 0.5; 5e-7 at bonds of 4; the FP64 path is exact),
+    # so this case is checked at 2e-3; FP64 is the precision for fields.
+    _run_parity(systems.chain(10, h=0.5), 16, 8, 17, steps=2, precision="fp32", tol=2e-3, check_states=False)
+
+
 def test_walk_from_trial_copies_fp64():
     # SetupWalkers with copyTrial (MPSQMC2.cpp:400-409): O = 1 at start
     s = systems.chain(8)
