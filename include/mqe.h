@@ -145,6 +145,16 @@ int mqe_last_kernel_ms(mqe_engine *e, double *ms, int *walkers);
 int mqe_set_timing(mqe_engine *e, int on);
 int mqe_kernel_time(mqe_engine *e, double *total_ms, int *launches);
 
+/* Snapshot of the walker population (tensors and environments as stored,
+ * weights, overlaps, energy histories, local energies, the re-orthonormalisation
+ * counter) and a caller step number, written to a file; mqe_load_state restores
+ * it into an engine created with the same parameters and trial.  A restored
+ * engine continues the walk exactly: the device random numbers are keyed by
+ * seed, step and population position.  (The reference has no resume;
+ * MPSQMC2.cpp:749-758 only writes the energy trace.) */
+int mqe_save_state(mqe_engine *e, const char *path, long step);
+int mqe_load_state(mqe_engine *e, const char *path, long *step);
+
 const char *mqe_last_error(void);
 
 /* ---- set-up helpers (host code; no GPU needed) ----------------------------- */

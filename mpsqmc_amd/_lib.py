@@ -75,6 +75,8 @@ SIGNATURES = {
     "mqe_set_timing": (c_int, [ctypes.c_void_p, c_int]),
     "mqe_kernel_time": (c_int, [ctypes.c_void_p, P(c_double), P(c_int)]),
     "mqe_last_error": (ctypes.c_char_p, []),
+    "mqe_save_state": (c_int, [ctypes.c_void_p, ctypes.c_char_p, c_long]),
+    "mqe_load_state": (c_int, [ctypes.c_void_p, ctypes.c_char_p, P(c_long)]),
     "mqe_bond_dims": (None, [c_int, c_int, c_int, P(c_int)]),
     "mqe_trotter_tables": (c_int, [c_int, c_double, c_double, P(c_double), P(c_double), P(c_double)]),
     "mqe_marsaglia_grid": (c_int, [c_int, c_int, P(c_double), P(c_int)]),

@@ -1748,4 +1748,129 @@ int mqe_last_kernel_ms(mqe_engine *e, double *ms, int *walkers) {
     return MQE_OK;
 }
 
+// ---- population snapshot ------------------------------------------------------
+// File: header (magic, sizes, counters), then per walker in population order its
+// tensors, right and left environments (as stored, in the engine's gauge), bond
+// dims and the four scalars.  Walkers are copied one at a time.
+struct StateHeader {
+    char magic[8];
+    int L, d, precision, tile32;
+    long wflat, eflat;
+    int n, prepared;
+    long prop_count, step;
+};
+
+int mqe_save_state(mqe_engine *e, const char *path, long step) {
+    if (!e || !path) { set_error("mqe_save_state: bad arguments"); return MQE_EINVAL; }
+    if (!e->loaded) { set_error("mqe_save_state: no walkers loaded"); return MQE_ESTATE; }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    int n = 0;
+    HIPCHK(hipMemcpy(&n, e->d_n, sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<int> order(std::max(n, 1));
+    HIPCHK(hipMemcpy(order.data(), e->d_order, sizeof(int) * n, hipMemcpyDeviceToHost));
+    FILE *f = std::fopen(path, "wb");
+    if (!f) { set_error("mqe_save_state: cannot open the file"); return MQE_EINVAL; }
+    StateHeader h;
+    std::memset(&h, 0, sizeof(h));
+    std::memcpy(h.magic, "MQESTAT1", 8);
+    h.L = e->L;
+    h.d = e->p.d;
+    h.precision = e->p.precision;
+    h.tile32 = e->tile32 ? 1 : 0;
+    h.wflat = e->wflat;
+    h.eflat = e->eflat;
+    h.n = n;
+    h.prepared = e->prepared ? 1 : 0;
+    h.prop_count = e->prop_count;
+    h.step = step;
+    const size_t tsz = e->fp32 ? sizeof(float) : sizeof(double);
+    std::vector<char> buf(tsz * (size_t)std::max(e->wflat, e->eflat));
+    std::vector<int> vd(e->L + 1);
+    bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1;
+    for (int b = 0; ok && b < n; ++b) {
+        const size_t sl = (size_t)order[b];
+        const struct { void *base; long len; } parts[3] = {{e->d_phi, e->wflat}, {e->d_Renv, e->eflat},
+                                                           {e->d_Lenv, e->eflat}};
+        for (const auto &pt : parts) {
+            if (hipMemcpy(buf.data(), (char *)pt.base + tsz * sl * pt.len, tsz * pt.len, hipMemcpyDeviceToHost) !=
+                hipSuccess) { std::fclose(f); set_error("mqe_save_state: device copy failed"); return MQE_EDEVICE; }
+            ok = ok && std::fwrite(buf.data(), tsz, pt.len, f) == (size_t)pt.len;
+        }
+        double sc[4];
+        HIPCHK(hipMemcpy(vd.data(), e->d_wvd + sl * (e->L + 1), sizeof(int) * (e->L + 1), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(&sc[0], e->d_w + sl, sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(&sc[1], e->d_O + sl, sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(&sc[2], e->d_eh + sl, sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(&sc[3], e->d_EL + sl, sizeof(double), hipMemcpyDeviceToHost));
+        ok = ok && std::fwrite(vd.data(), sizeof(int), e->L + 1, f) == (size_t)(e->L + 1);
+        ok = ok && std::fwrite(sc, sizeof(double), 4, f) == 4;
+    }
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) { set_error("mqe_save_state: write failed"); return MQE_EINVAL; }
+    return MQE_OK;
+}
+
+int mqe_load_state(mqe_engine *e, const char *path, long *step) {
+    if (!e || !path) { set_error("mqe_load_state: bad arguments"); return MQE_EINVAL; }
+    FILE *f = std::fopen(path, "rb");
+    if (!f) { set_error("mqe_load_state: cannot open the file"); return MQE_EINVAL; }
+    StateHeader h;
+    if (std::fread(&h, sizeof(h), 1, f) != 1 || std::memcmp(h.magic, "MQESTAT1", 8) != 0) {
+        std::fclose(f);
+        set_error("mqe_load_state: not a walker-state file");
+        return MQE_EINVAL;
+    }
+    if (h.L != e->L || h.d != e->p.d || h.precision != e->p.precision || h.tile32 != (e->tile32 ? 1 : 0) ||
+        h.wflat != e->wflat || h.eflat != e->eflat) {
+        std::fclose(f);
+        set_error("mqe_load_state: the file was written by an engine with other parameters");
+        return MQE_EINVAL;
+    }
+    if (h.n < 1 || h.n > e->p.capacity) {
+        std::fclose(f);
+        set_error("mqe_load_state: population size exceeds the capacity");
+        return MQE_ECAPACITY;
+    }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    const size_t tsz = e->fp32 ? sizeof(float) : sizeof(double);
+    std::vector<char> buf(tsz * (size_t)std::max(e->wflat, e->eflat));
+    std::vector<int> vd(e->L + 1);
+    bool ok = true;
+    for (int b = 0; ok && b < h.n; ++b) {
+        const size_t sl = (size_t)b;
+        const struct { void *base; long len; } parts[3] = {{e->d_phi, e->wflat}, {e->d_Renv, e->eflat},
+                                                           {e->d_Lenv, e->eflat}};
+        for (const auto &pt : parts) {
+            ok = ok && std::fread(buf.data(), tsz, pt.len, f) == (size_t)pt.len;
+            if (!ok) break;
+            HIPCHK(hipMemcpy((char *)pt.base + tsz * sl * pt.len, buf.data(), tsz * pt.len, hipMemcpyHostToDevice));
+        }
+        double sc[4];
+        ok = ok && std::fread(vd.data(), sizeof(int), e->L + 1, f) == (size_t)(e->L + 1);
+        ok = ok && std::fread(sc, sizeof(double), 4, f) == 4;
+        if (!ok) break;
+        for (int k = 0; k <= e->L; ++k)
+            if (vd[k] < 1 || vd[k] > std::max(e->Dw, 1) || (e->tile32 && vd[k] > 32)) ok = false;
+        if (!ok) break;
+        HIPCHK(hipMemcpy(e->d_wvd + sl * (e->L + 1), vd.data(), sizeof(int) * (e->L + 1), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(e->d_w + sl, &sc[0], sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(e->d_O + sl, &sc[1], sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(e->d_eh + sl, &sc[2], sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(e->d_EL + sl, &sc[3], sizeof(double), hipMemcpyHostToDevice));
+    }
+    std::fclose(f);
+    if (!ok) { set_error("mqe_load_state: truncated or inconsistent file"); return MQE_EINVAL; }
+    std::vector<int> ord(e->p.capacity);
+    for (int i = 0; i < e->p.capacity; ++i) ord[i] = i;
+    HIPCHK(hipMemcpy(e->d_order, ord.data(), sizeof(int) * ord.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->d_n, &h.n, sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(e->d_status, 0, sizeof(int)));
+    e->n_host = h.n;
+    e->prop_count = h.prop_count;
+    e->loaded = true;
+    e->prepared = h.prepared != 0;
+    if (step) *step = h.step;
+    return MQE_OK;
+}
+
 }  // extern "C"

@@ -6,6 +6,7 @@ The method names follow the reference driver that the engine replaces:
 ``population_control`` = MPSQMC2::SeparatePopulationControl (:613-686).
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -127,6 +128,18 @@ class Engine:
 
     def set_canonical_interval(self, steps):
         check(lib().mqe_set_canonical_interval(self.h, steps))
+
+    # -- checkpoint -----------------------------------------------------------
+    def save_state(self, path, step=0):
+        """Write the population snapshot (mqe_save_state) with a step number."""
+        check(lib().mqe_save_state(self.h, os.fsencode(path), int(step)))
+
+    def load_state(self, path):
+        """Restore a snapshot written by an engine with the same parameters; returns its step."""
+        st = ctypes.c_long()
+        check(lib().mqe_load_state(self.h, os.fsencode(path), ctypes.byref(st)))
+        self.n = self.num_walkers()
+        return st.value
 
     @property
     def kernel_name(self):

@@ -116,6 +116,8 @@ def test_engine_calls_fail_cleanly_without_a_device():
     assert lib.mqe_prepare(None, ctypes.byref(d)) < 0
     assert lib.mqe_propagate(None, None, ctypes.byref(d)) < 0
     assert lib.mqe_trotter_tables(3, 1.0, 0.01, None, None, None) == -1
+    assert lib.mqe_save_state(None, b"/tmp/x", 0) == -1
+    assert lib.mqe_load_state(None, b"/tmp/x", None) == -1
 
 
 def test_snake_lattice_bonds():
