@@ -171,6 +171,12 @@ class MPSQMC2 {
     // Let the walkers propagate for steps steps (MPSQMC2.cpp:443-513)
     void Walk(int steps);
 
+    // Walker population snapshot and exact resume (not in the reference): the
+    // next Walk continues from the saved step.  With the host random stream the
+    // caller restores that stream itself.
+    void saveCheckpoint(const std::string &path) const;
+    void loadCheckpoint(const std::string &path);
+
     // trace file: the reference writes output/Heisenberg1D/ener_L%dDT%dDW%d.txt (MPSQMC2.cpp:452,752)
     void setTraceFile(const std::string &path) { trace_path_ = path; }
     void setVerbose(bool v) { verbose_ = v; }

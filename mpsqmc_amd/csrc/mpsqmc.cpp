@@ -396,6 +396,20 @@ Walker MPSQMC2::walker(int i) const {
     return Walker(&st, O[i], w[i], eh[i]);
 }
 
+void MPSQMC2::saveCheckpoint(const std::string &path) const {
+    ck(mqe_save_state(eng_, path.c_str(), step_offset_), "MPSQMC2::saveCheckpoint");
+}
+
+void MPSQMC2::loadCheckpoint(const std::string &path) {
+    long st = 0;
+    ck(mqe_load_state(eng_, path.c_str(), &st), "MPSQMC2::loadCheckpoint");
+    int n = 0;
+    ck(mqe_num_walkers(eng_, &n), "MPSQMC2::loadCheckpoint");
+    totalNCurrentWalkers_ = n;
+    step_offset_ = (int)st;
+    started_ = true;
+}
+
 void MPSQMC2::Walk(int steps) {
     double projectedEnergy = 0.0;
     if (!started_) {
