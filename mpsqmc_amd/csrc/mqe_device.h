@@ -29,6 +29,12 @@ __device__ __forceinline__ double dpp_d(double x) {
     return __hiloint2double(hi2, lo2);
 }
 
+__device__ __forceinline__ double readlane_d_(double v, int l) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
+
 // sum over aligned groups of 8 lanes (xor 1, xor 2 inside quads, then the
 // mirrored half-row), every lane of the group receives the sum
 __device__ __forceinline__ double group8_sum(double v) {
@@ -38,33 +44,40 @@ __device__ __forceinline__ double group8_sum(double v) {
     return v;
 }
 
-// sum over the wave64; every lane receives the total
-__device__ __forceinline__ double wave_sum(double v) {
-    v = group8_sum(v);
-    v += dpp_d<0x140>(v);  // row_mirror: the two 8-lane halves of a 16-lane row
-    v += __shfl_xor(v, 16, 64);
-    v += __shfl_xor(v, 32, 64);
+// DPP move of a double with a row mask; lanes of disabled rows receive 0
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_dm(double x) {
+    const int lo = __double2loint(x), hi = __double2hiint(x);
+    const int lo2 = __builtin_amdgcn_update_dpp(0, lo, CTRL, ROWS, 0xF, false);
+    const int hi2 = __builtin_amdgcn_update_dpp(0, hi, CTRL, ROWS, 0xF, false);
+    return __hiloint2double(hi2, lo2);
+}
+
+// inclusive prefix sum over the 64 lanes with DPP only: row_shr 1, 2, 4, 8
+// inside each 16-lane row, then row_bcast:15 (rows 1, 3) and row_bcast:31
+// (rows 2, 3) carry the row totals (lane semantics checked on gfx950)
+__device__ __forceinline__ double wave_inclusive_scan(double v) {
+    v += dpp_d<0x111>(v);
+    v += dpp_d<0x112>(v);
+    v += dpp_d<0x114>(v);
+    v += dpp_d<0x118>(v);
+    v += dpp_dm<0x142, 0xA>(v);
+    v += dpp_dm<0x143, 0xC>(v);
     return v;
 }
 
-// value of v in lane l (l wave-uniform)
-__device__ __forceinline__ double readlane_d(double v, int l) {
-    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
-    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
-    return __hiloint2double(hi, lo);
+// sum over the wave64; every lane receives the total (lane 63 of the scan)
+__device__ __forceinline__ double wave_sum(double v) {
+    return readlane_d_(wave_inclusive_scan(v), 63);
 }
 
-// exclusive prefix sum over the lanes of the wave64 (lane 0 receives 0)
+// value of v in lane l (l wave-uniform)
+__device__ __forceinline__ double readlane_d(double v, int l) { return readlane_d_(v, l); }
+
+// exclusive prefix sum over the lanes of the wave64 (lane 0 receives 0):
+// the inclusive scan shifted by one lane (DPP wave_shr:1)
 __device__ __forceinline__ double wave_exclusive_scan(double v) {
-    const int lane = threadIdx.x & 63;
-    double s = v;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const double o = __shfl_up(s, off, 64);
-        if (lane >= off) s += o;
-    }
-    const double e = __shfl_up(s, 1, 64);
-    return lane == 0 ? 0.0 : e;
+    return dpp_d<0x138>(wave_inclusive_scan(v));
 }
 
 // Sum over the workgroup; every thread receives the total.  sh: >= MQE_WAVES doubles.

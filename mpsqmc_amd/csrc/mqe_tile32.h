@@ -87,19 +87,19 @@ __device__ __forceinline__ void mma_frag(v16f &acc, const frag16 &a, const float
 // MFMA result (accumulator layout: slot q of lane (j, h) holds row
 // 8(q/4) + 4h + q%4 of column j) -> B-operand fragment of the same matrix
 // (b[q] = row 2q + h of column j): half of the rows move between lanes l and
-// l ^ 32.
+// l ^ 32.  v_permlane32_swap(x, y) returns (x of lanes 0-31 | y of lanes 0-31)
+// and (x of lanes 32-63 | y of lanes 32-63) (checked on gfx950), which is
+// exactly (b[q], b[q|2]) for x = acc[base], y = acc[base + 1].
 __device__ __forceinline__ void acc_to_bfrag(const v16f &acc, frag16 &b) {
-    const int h = (threadIdx.x & 63) >> 5;
 #pragma unroll
     for (int g = 0; g < 4; ++g)
 #pragma unroll
         for (int o = 0; o < 2; ++o) {
             const int base = 4 * g + 2 * o, q = 4 * g + o;
-            const float keep = h ? acc[base + 1] : acc[base];
-            const float send = h ? acc[base] : acc[base + 1];
-            const float recv = __shfl_xor(send, 32, 64);
-            b[q] = h ? recv : keep;
-            b[q | 2] = h ? keep : recv;
+            const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(acc[base]),
+                                                            __float_as_uint(acc[base + 1]), false, false);
+            b[q] = __uint_as_float(r[0]);
+            b[q | 2] = __uint_as_float(r[1]);
         }
 }
 
@@ -162,14 +162,21 @@ __device__ __forceinline__ float dot_partial(const v16f &acc, const float *B, in
     return f;
 }
 
+// (x, y) -> x + y of the partner half: for lanes 0-31 x(l) + x(l + 32), for
+// lanes 32-63 y(l - 32) + y(l), with one v_permlane32_swap per dword
+__device__ __forceinline__ double halves_sum(double x, double y) {
+    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)__double2loint(x), (unsigned)__double2loint(y),
+                                                     false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)__double2hiint(x), (unsigned)__double2hiint(y),
+                                                     false, false);
+    return __hiloint2double((int)hi[0], (int)lo[0]) + __hiloint2double((int)hi[1], (int)lo[1]);
+}
+
 // four wave-wide sums: lane l receives the sum of s_{l >> 4}
 __device__ __forceinline__ double reduce4(double s0, double s1, double s2, double s3) {
     const int lane = threadIdx.x & 63;
-    const bool hi = lane >= 32, h16 = (lane & 16) != 0;
-    double a0 = hi ? s2 : s0, a1 = hi ? s3 : s1;
-    const double b0 = hi ? s0 : s2, b1 = hi ? s1 : s3;
-    a0 += __shfl_xor(b0, 32, 64);
-    a1 += __shfl_xor(b1, 32, 64);
+    const bool h16 = (lane & 16) != 0;
+    const double a0 = halves_sum(s0, s2), a1 = halves_sum(s1, s3);  // lanes < 32: s0, s1; else s2, s3
     double c0 = h16 ? a1 : a0;
     const double c1 = h16 ? a0 : a1;
     c0 += __shfl_xor(c1, 16, 64);
@@ -183,26 +190,8 @@ __device__ __forceinline__ double reduce4(double s0, double s1, double s2, doubl
 // sum is FP64.  Lane l receives dot l >> 4 (a reduce-scatter over the lane
 // halves, then a DPP sum over each 16-lane row).
 __device__ __forceinline__ double dot4_tile(const v16f &acc, const float *base, int ldc) {
-    const int j = col_of(), lane = threadIdx.x & 63;
-    double s[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        float f = 0.f;
-#pragma unroll
-        for (int e = 0; e < 16; ++e) f += acc[e] * base[q * BLK + row_of(e) + ldc * j];
-        s[q] = (double)f;
-    }
-    const bool hi = lane >= 32, h16 = (lane & 16) != 0;
-    double a0 = hi ? s[2] : s[0], a1 = hi ? s[3] : s[1];
-    const double b0 = hi ? s[0] : s[2], b1 = hi ? s[1] : s[3];
-    a0 += __shfl_xor(b0, 32, 64);
-    a1 += __shfl_xor(b1, 32, 64);
-    double c0 = h16 ? a1 : a0;
-    const double c1 = h16 ? a0 : a1;
-    c0 += __shfl_xor(c1, 16, 64);
-    c0 = group8_sum(c0);
-    c0 += dpp_d<0x140>(c0);
-    return c0;
+    return reduce4(dot_partial(acc, base, ldc), dot_partial(acc, base + BLK, ldc),
+                   dot_partial(acc, base + 2 * BLK, ldc), dot_partial(acc, base + 3 * BLK, ldc));
 }
 
 // global padded block -> LDS block (whole workgroup)
