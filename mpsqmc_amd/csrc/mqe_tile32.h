@@ -172,14 +172,21 @@ __device__ __forceinline__ double halves_sum(double x, double y) {
     return __hiloint2double((int)hi[0], (int)lo[0]) + __hiloint2double((int)hi[1], (int)lo[1]);
 }
 
+// (x, y) -> for 16-lane rows 0 and 2 x(l) + x(l + 16), for rows 1 and 3
+// y(l - 16) + y(l), with one v_permlane16_swap per dword (it returns rows
+// (x0, y0, x2, y2) and (x1, y1, x3, y3); checked on gfx950)
+__device__ __forceinline__ double rows_sum(double x, double y) {
+    const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)__double2loint(x), (unsigned)__double2loint(y),
+                                                     false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)__double2hiint(x), (unsigned)__double2hiint(y),
+                                                     false, false);
+    return __hiloint2double((int)hi[0], (int)lo[0]) + __hiloint2double((int)hi[1], (int)lo[1]);
+}
+
 // four wave-wide sums: lane l receives the sum of s_{l >> 4}
 __device__ __forceinline__ double reduce4(double s0, double s1, double s2, double s3) {
-    const int lane = threadIdx.x & 63;
-    const bool h16 = (lane & 16) != 0;
     const double a0 = halves_sum(s0, s2), a1 = halves_sum(s1, s3);  // lanes < 32: s0, s1; else s2, s3
-    double c0 = h16 ? a1 : a0;
-    const double c1 = h16 ? a0 : a1;
-    c0 += __shfl_xor(c1, 16, 64);
+    double c0 = rows_sum(a0, a1);                                    // row r: s_r over four lanes
     c0 = group8_sum(c0);
     c0 += dpp_d<0x140>(c0);
     return c0;
