@@ -47,6 +47,7 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=20131015)
+    ap.add_argument("--random-trial", action="store_true", help="random trial instead of the DMRG one")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB ten times (reference for HBM counters)")
     return ap.parse_args()
@@ -81,11 +82,17 @@ def measured_traffic(L, D, N):
     return None
 
 
-def make_trial(L, D, seed):
+def make_trial(system, D, seed, random_trial=False):
+    """The trial state: two-site DMRG of the chain in the D bond profile (the
+    reference's MPSQMC2 DT constructor does the same), or a random
+    left-normalised MPS of that profile."""
     from mpsqmc_amd import mps as M
-    vd, data = M.random_mps(L, D, 2, seed)
-    data, _ = M.left_normalize(vd, data)
-    return vd, data
+    if random_trial:
+        vd, data = M.random_mps(system.L, D, 2, seed)
+        data, _ = M.left_normalize(vd, data)
+        return vd, data, None
+    vd, data, E = M.dmrg(system, D, max_sweeps=6, tol=1e-8, seed=seed)
+    return vd, data, E
 
 
 def cpu_baseline(system, vd, data, walkers, threads):
@@ -123,7 +130,10 @@ def main():
 
     L, D, N = args.L, args.D, args.walkers
     system = systems.chain(L, J=1.0, dtau=0.01)
-    vd, data = make_trial(L, D, args.seed)
+    t0 = time.perf_counter()
+    vd, data, E_trial = make_trial(system, D, args.seed, args.random_trial)
+    log(f"bench: trial ready ({'random' if E_trial is None else f'DMRG, E = {E_trial:.10f}'}, "
+        f"{time.perf_counter() - t0:.1f} s)")
     eng = Engine(system, vd, data, n_desired=N, capacity=3 * N, max_bond=D, precision=args.precision,
                  seed=args.seed)
     eng.load_copies(vd, data, N)
@@ -166,7 +176,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64" if args.precision == "fp32" else "f64",
-        "data": "synthetic: random left-normalized trial MPS with the D=32 bond profile; walkers start as copies",
+        "data": ("synthetic: random left-normalized trial MPS with the D bond profile" if E_trial is None else
+                 "synthetic: trial from the built-in two-site DMRG of the chain (D bond profile)") +
+                "; walkers start as copies of the trial",
         "config": {"workload": f"heisenberg-chain L={L} OBC J=1, trial D={D}, walkers D={D}, N={N}, dtau=0.01, "
                                f"Marsaglia(4) grid, {args.precision} contractions",
                    "walkers": N, "L": L, "D": D},
@@ -174,6 +186,7 @@ def main():
                      "frac": achieved / peak, "traffic": measured_traffic(L, D, N),
                      "kernel": eng.kernel_name, "avg_launch_ms": avg_launch_s * 1000.0,
                      "flops_per_walker_step": F32_PER_WS + F64_PER_WS},
+        "E_trial": E_trial,
         "E_P_last": float(trace[-1, 2]),
         "population_last": int(trace[-1, 1]),
     }
