@@ -434,76 +434,87 @@ __device__ double energy_sweep32(const W32 &C, int pc, double *On, double *Ou, b
         // The last site's propagated tensor is only in LDS: store it.
         l2g(C.blk(pc), C.ph(L - 1, 0));
         l2g(C.blk(pc) + BLK, C.ph(L - 1, 1));
-        float pre[16];  // phi_{k-1} and L_{k-1}, fetched one site ahead
+        // operands of site k, stored one site ahead: phi_{k-1} (into the buffer
+        // that becomes free) and L_{k-1}
+        float pre[16];
         {
             const float *const g[3] = {C.ph(L - 2, 0), C.ph(L - 2, 1), C.Lg + (size_t)(L - 2) * GB};
             pre_load<3>(pre, g);
+            float *const d[3] = {C.blk(pj), C.blk(pj) + BLK, C.blk(B_L)};
+            pre_store<3>(pre, d);
         }
+        // trial operands: T_k[0], T_k[1] (waves 0,1, stage 1) and T_{k-1}[s']^T (stage 2)
+        frag16 f0, f1, fg;
+        if (w < 2) {
+            load_frag(f0, C.tn(L - 1, 0));
+            load_frag(f1, C.tn(L - 1, 1));
+        }
+        load_frag(fg, C.tt(L - 2, w >> 1));
+        __syncthreads();
         for (int k = L - 1; k >= 1; --k) {
             float *cen = C.blk(pc), *nxt = C.blk(pj);
-            frag16 fv, fg;  // trial operands of stages 2 and 4
-            load_frag(fv, C.tn(k, w >> 1));
-            load_frag(fg, C.tt(k - 1, w >> 1));
-            {
-                float *const d[3] = {nxt, nxt + BLK, C.blk(B_L)};
-                pre_store<3>(pre, d);
-            }
-            __syncthreads();
             if (k >= 2) {
                 const float *const g[3] = {C.ph(k - 2, 0), C.ph(k - 2, 1), C.Lg + (size_t)(k - 2) * GB};
                 pre_load<3>(pre, g);
             }
-            {  // stage 1: W[t] = R_{k+1} phi_k[t]^T (waves 0,1); X[t] = phi_k[t] M_{k+1} (waves 2,3)
-                v16f acc = zero16();
-                if (w < 2) {
-                    mma<false, true>(acc, C.blk(B_R), LD, cen + w * BLK, LD);
-                    store(acc, C.blk(B_W) + w * BLK, LD);
-                } else {
-                    mma<false, false>(acc, cen + (w - 2) * BLK, LD, C.blk(B_LM), LD);
-                    store(acc, C.blk(B_Y) + (w - 2) * BLK, LD);
-                }
+            // stage 1, chained in registers:
+            //   waves 0,1 (t = w):     W[t] = R_{k+1} phi_k[t]^T,  V(t',t) = T_k[t'] W[t]  (to B_V + 2t' + t)
+            //   waves 2,3 (t = w - 2): Z[t] = M_{k+1} phi_k[t]^T,  phi_k[t] Z[t]          (term t of M_k, to B_W + t)
+            //                          Y[t] = L_{k-1} phi_{k-1}[t]                         (to B_Y + t)
+            if (w < 2) {
+                v16f y = zero16(), r0 = zero16(), r1 = zero16();
+                mma<false, true>(y, C.blk(B_R), LD, cen + w * BLK, LD);
+                frag16 yb;
+                acc_to_bfrag(y, yb);
+                mma_rr(r0, f0, yb);
+                mma_rr(r1, f1, yb);
+                store(r0, C.blk(B_V) + w * BLK, LD);
+                store(r1, C.blk(B_V) + (2 + w) * BLK, LD);
+            } else {
+                const int t = w - 2;
+                v16f z = zero16(), m = zero16(), y = zero16();
+                mma<false, true>(z, C.blk(B_LM), LD, cen + t * BLK, LD);
+                frag16 zb;
+                acc_to_bfrag(z, zb);
+                mma_lr<false>(m, cen + t * BLK, LD, zb);
+                store(m, C.blk(B_W) + t * BLK, LD);
+                mma<false, false>(y, C.blk(B_L), LD, nxt + t * BLK, LD);
+                store(y, C.blk(B_Y) + t * BLK, LD);
             }
             __syncthreads();
-            {  // stage 2: V(t',t) = T_k[t'] W[t]
-                v16f acc = zero16();
-                mma_frag(acc, fv, C.blk(B_W) + (w & 1) * BLK, LD);
-                store(acc, C.blk(B_V) + w * BLK, LD);
-            }
-            __syncthreads();
-            {  // stage 3: R_k = V(0,0) + V(1,1); Y[s] = L_{k-1} phi_{k-1}[s] (waves 0,1, into B_W);
-               // the two terms X[t] phi_k[t]^T of M_k (waves 2,3, into B_LM and B_N)
-                const float *V = C.blk(B_V);
-                float *R = C.blk(B_R);
-                for (int e = threadIdx.x; e < BLK; e += MQE_NT) R[e] = V[e] + V[3 * BLK + e];
-                v16f acc = zero16();
-                if (w < 2) {
-                    mma<false, false>(acc, C.blk(B_L), LD, nxt + w * BLK, LD);
-                    store(acc, C.blk(B_W) + w * BLK, LD);
-                } else {
-                    mma<false, true>(acc, C.blk(B_Y) + (w - 2) * BLK, LD, cen + (w - 2) * BLK, LD);
-                    store(acc, C.blk(w == 2 ? B_LM : B_N), LD);
-                }
-            }
-            __syncthreads();
-            {  // stage 4: bond energy of (k-1, k); M_k = sum of its two terms; environments to global memory
-                const float *V = C.blk(B_V);
+            // stage 2: bond energy of (k-1, k) from G(s',s) = T_{k-1}[s']^T Y[s] (wave w: s' = w >> 1,
+            // s = w & 1) and V; R_k = V(0,0) + V(1,1) and M_k = sum of its terms to LDS and global
+            // memory; the next site's operands
+            {
+                const float *V = C.blk(B_V), *Mt = C.blk(B_W);
                 v16f g = zero16();
-                mma_frag(g, fg, C.blk(B_W) + (w & 1) * BLK, LD);
+                mma_frag(g, fg, C.blk(B_Y) + (w & 1) * BLK, LD);
+                if (k >= 2) {
+                    if (w < 2) {
+                        load_frag(f0, C.tn(k - 1, 0));
+                        load_frag(f1, C.tn(k - 1, 1));
+                    }
+                    load_frag(fg, C.tt(k - 2, w >> 1));
+                }
                 const int sp = w >> 1, ss = w & 1;
                 {
                     const double d = dot4_tile(g, V, LD);
                     const int q = lane >> 4;
                     if ((lane & 15) == 0) Mq[(sp + 2 * (q >> 1)) * 4 + (ss + 2 * (q & 1))] = d;
                 }
-                l2g(C.blk(B_R), C.Rg + (size_t)k * GB);
-                float *Mk = C.blk(B_LM);
-                const float *Mb = C.blk(B_N);
-                float *mg = C.Rg + (size_t)(L + 1 + k) * GB;
+                float *R = C.blk(B_R), *Mk = C.blk(B_LM);
+                float *rg = C.Rg + (size_t)k * GB, *mg = C.Rg + (size_t)(L + 1 + k) * GB;
                 for (int e = threadIdx.x; e < GB; e += MQE_NT) {
                     const int o = (e & 31) + LD * (e >> 5);
-                    const float v = Mk[o] + Mb[o];
-                    Mk[o] = v;
-                    mg[e] = v;
+                    const float rv = V[o] + V[3 * BLK + o], mv = Mt[o] + Mt[BLK + o];
+                    R[o] = rv;
+                    rg[e] = rv;
+                    Mk[o] = mv;
+                    mg[e] = mv;
+                }
+                if (k >= 2) {  // phi_{k-2} into the buffer of phi_k (not read in this stage), L_{k-2}
+                    float *const d[3] = {cen, cen + BLK, C.blk(B_L)};
+                    pre_store<3>(pre, d);
                 }
             }
             __syncthreads();
