@@ -697,9 +697,9 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
         pre_load<4>(pre, g);
     }
     for (int e = threadIdx.x; e < 4 * Ng; e += MQE_NT) grid_s[e] = S.grid[e];
-    // tables of coupling c: tab[0..3] = S, tab[4..19] = U, tab[20..35] = VT
+    // tables of coupling c: tab[0..3] = sqrt(S), tab[4..19] = U, tab[20..35] = VT
     auto table_entry = [&](int c, int t) {
-        return t < 4 ? S.tS[4 * c + t] : (t < 20 ? S.tU[16 * c + t - 4] : S.tVT[16 * c + t - 20]);
+        return t < 4 ? sqrt(S.tS[4 * c + t]) : (t < 20 ? S.tU[16 * c + t - 4] : S.tVT[16 * c + t - 20]);
     };
     if (threadIdx.x < 36) tab_s[0][threadIdx.x] = table_entry(0, threadIdx.x);
     double tnext = (threadIdx.x < 36 && S.nC > 1) ? table_entry(1, threadIdx.x) : 0.0;
@@ -775,20 +775,32 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
         // stage D: combos, PDF, CDF sample (MPSQMC2.cpp:540-588).  Every wave
         // computes the same result, so no workgroup barrier is needed: lane
         // q < 16 forms combo q, lane l owns grid points 4l .. 4l+3.
-        const double *tS = tab_s[c & 1], *tU = tS + 4, *tVT = tS + 20;
+        const double *sS = tab_s[c & 1], *tU = sS + 4, *tVT = sS + 20;  // sS = sqrt(S)
         double cq = 0.0;
         if (lane < 16) {
+            // combo (a, b) = sqrt(S_a S_b) sum U_a(s',s) VT_b(t',t) M[(s'+2t')4 + (s+2t)], operands loaded first
             const int a = lane & 3, bb = lane >> 2;
+            double u[4], v[4], m[16];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                u[q] = tU[(q & 1) + 2 * ((q >> 1) + 2 * a)];    // u[s' + 2s] = U_a(s', s)
+                v[q] = tVT[bb + 4 * ((q & 1) + 2 * (q >> 1))];  // v[t' + 2t] = VT_b(t', t)
+            }
+#pragma unroll
+            for (int q = 0; q < 16; ++q) m[q] = M[q];
             double sacc = 0.0;
+#pragma unroll
             for (int xsp = 0; xsp < 2; ++xsp)
+#pragma unroll
                 for (int xss = 0; xss < 2; ++xss) {
-                    const double ua = tU[xsp + 2 * (xss + 2 * a)];
-                    if (ua == 0.0) continue;
+                    double mv = 0.0;
+#pragma unroll
                     for (int tp = 0; tp < 2; ++tp)
-                        for (int t = 0; t < 2; ++t)
-                            sacc += ua * tVT[bb + 4 * (tp + 2 * t)] * M[(xsp + 2 * tp) * 4 + (xss + 2 * t)];
+#pragma unroll
+                        for (int t = 0; t < 2; ++t) mv += v[tp + 2 * t] * m[(xsp + 2 * tp) * 4 + (xss + 2 * t)];
+                    sacc += u[xsp + 2 * xss] * mv;
                 }
-            cq = sqrt(tS[a] * tS[bb]) * (sacc * sc);
+            cq = sS[a] * sS[bb] * (sacc * sc);
         }
         double cb[16];
 #pragma unroll
@@ -842,7 +854,7 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
         double Ux[4] = {0.0, 0.0, 0.0, 0.0}, Vx[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
-            const double pf = sqrt(tS[kk] * 4.0) * xs[kk];
+            const double pf = 2.0 * sS[kk] * xs[kk];  // sqrt(4 S_k) x_k (MPSstate.cpp:591,616)
 #pragma unroll
             for (int up = 0; up < 2; ++up)
 #pragma unroll
