@@ -813,11 +813,14 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
             double v = 0.0;
             if (gq < Ng) {
                 const double *x = grid_s + 4 * gq;
-                double f = 0.0;
+                double f = 0.0;  // sum_l x_l (sum_r comb[l + 4r] x_r)
 #pragma unroll
-                for (int l = 0; l < 4; ++l)
+                for (int l = 0; l < 4; ++l) {
+                    double t = 0.0;
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) f += cb[l + 4 * r] * x[l] * x[r];
+                    for (int r = 0; r < 4; ++r) t += cb[l + 4 * r] * x[r];
+                    f += x[l] * t;
+                }
                 v = (f > 0.0 ? f : 0.0) * pref;
             }
             pv[t] = v;
@@ -848,9 +851,12 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
         const double *xs = grid_s + 4 * gs;
         fstar = 0.0;
 #pragma unroll
-        for (int l = 0; l < 4; ++l)
+        for (int l = 0; l < 4; ++l) {
+            double t = 0.0;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) fstar += cb[l + 4 * r] * xs[l] * xs[r];
+            for (int r = 0; r < 4; ++r) t += cb[l + 4 * r] * xs[r];
+            fstar += xs[l] * t;
+        }
         double Ux[4] = {0.0, 0.0, 0.0, 0.0}, Vx[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
