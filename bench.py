@@ -9,6 +9,10 @@ every walker, the weight/energy reductions and population control.  Inputs are
 resident in HBM before the timed region; the device RNG supplies the
 auxiliary-field draws.
 
+The second figure of the metric, variance x GPU-s, is the blocking-analysis
+variance of the mean projected energy over the timed steps times their wall
+time (the cost of a run at a fixed statistical error is proportional to it).
+
 Prints one JSON line (see the keys below).
 """
 import argparse
@@ -37,8 +41,8 @@ def log(msg):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=64)
-    ap.add_argument("--warmup", type=int, default=32)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--walkers", type=int, default=2048)
     ap.add_argument("--L", type=int, default=32)
     ap.add_argument("--D", type=int, default=32)
@@ -157,6 +161,8 @@ def main():
     walker_steps = float(trace[:, 1].sum())
     wall = t1 - t0
     value = walker_steps / wall
+    from mpsqmc_amd.analysis import variance_time
+    vt = variance_time(trace[:, 2], wall) if args.steps >= 64 else None
     # roofline of the dominant kernel (the fused walker kernel)
     avg_launch_s = ktot_ms / 1000.0 / max(nlaunch, 1)
     walkers_per_launch = walker_steps / max(nlaunch, 1)
@@ -186,6 +192,10 @@ def main():
                      "frac": achieved / peak, "traffic": measured_traffic(L, D, N),
                      "kernel": eng.kernel_name, "avg_launch_ms": avg_launch_s * 1000.0,
                      "flops_per_walker_step": F32_PER_WS + F64_PER_WS},
+        "variance_x_gpu_s": None if vt is None else {
+            "value": vt["variance_x_seconds"], "unit": "E^2*s", "proposal": "plain",
+            "E_mean": vt["mean"], "E_error": vt["error"], "block_length": vt["block_length"],
+            "steps": vt["n"], "seconds": vt["seconds"]},
         "E_trial": E_trial,
         "E_P_last": float(trace[-1, 2]),
         "population_last": int(trace[-1, 1]),

@@ -157,6 +157,70 @@ int mqe_load_state(mqe_engine *e, const char *path, long *step);
 
 const char *mqe_last_error(void);
 
+/* ---- learned proposal -------------------------------------------------------
+ * The reference has no learned part (its proposal is the exact PDF,
+ * MPSQMC2.cpp:552-588); these calls have no counterpart there.
+ *
+ * Features of walker b at coupling c (MQE_NFEAT floats, from the start of the
+ * step): log O, log w, E_L, e_{c-1}, e_c, e_{c+1}, c/(nC-1), with e_k the
+ * local energy of bond k from the previous energy sweep (0 beyond the ends).
+ * Labels: the 16 normalised two-site elements combos[a+4b]/O of the coupling
+ * (MPSQMC2.cpp:540-549), so that ratio(x) = sum_ab label[a+4b] x_a x_b.
+ *
+ * Learned proposal: with predicted labels p, q(x) = (1-mix) qhat(x)/sum qhat +
+ * mix/N_grid, qhat(x) = max(0, sum_ab p[a+4b] x_a x_b) (uniform when all qhat
+ * are 0); x is drawn from q by the reference's CDF scan and the weight is
+ * multiplied by pdf(x)/q(x), pdf(x) = max(0, ratio(x)) 4/N_grid (the
+ * reference's unnormalised PDF), so E[weight] is the reference's w * PDFnorm
+ * (unbiased).  If pdf(x) = 0 the weight becomes 0 and the walker continues
+ * with the reference's own sample.  Tile path only (nearest-neighbour chains,
+ * bonds <= 32, FP32). */
+#define MQE_NFEAT 7
+#define MQE_NLABEL 16
+
+typedef struct mqe_mlp mqe_mlp;
+
+/* MLP n_in -> hidden -> hidden -> n_out with ReLU, FP32, He-normal weights
+ * drawn from std::mt19937_64(seed).  n_in <= 16, hidden <= 128, n_out <= 32. */
+int mqe_mlp_create(int n_in, int hidden, int n_out, unsigned long long seed, mqe_mlp **out);
+void mqe_mlp_destroy(mqe_mlp *m);
+long mqe_mlp_num_params(const mqe_mlp *m);
+/* parameters: W1 [hidden x n_in], b1, W2 [hidden x hidden], b2, W3 [n_out x hidden], b3 (column-major) */
+int mqe_mlp_get_params(mqe_mlp *m, float *p);
+int mqe_mlp_set_params(mqe_mlp *m, const float *p);   /* also resets the Adam state */
+/* input/output standardisation: x' = (x - x_mean)/x_std, y = y' y_std + y_mean */
+int mqe_mlp_get_scaling(mqe_mlp *m, float *x_mean, float *x_std, float *y_mean, float *y_std);
+int mqe_mlp_set_scaling(mqe_mlp *m, const float *x_mean, const float *x_std, const float *y_mean,
+                        const float *y_std);
+/* Y[n x n_out] = net(X[n x n_in]) (row-major host arrays) */
+int mqe_mlp_forward(mqe_mlp *m, int n, const float *X, float *Y);
+/* mean squared error of the standardised outputs over n rows and its gradient
+ * with respect to the parameters (num_params floats) */
+int mqe_mlp_loss_grad(mqe_mlp *m, int n, const float *X, const float *Y, double *loss, float *grad);
+/* Adam (lr, 0.9, 0.999, 1e-8) over `epochs` seeded shuffles in minibatches of
+ * `batch` rows; standardize != 0 first sets the scaling from the data.
+ * *loss: the last epoch's mean loss. */
+int mqe_mlp_fit(mqe_mlp *m, int n, const float *X, const float *Y, int epochs, int batch, double lr,
+                int standardize, double *loss);
+
+/* Collect (feature, label) rows during the following propagations, up to
+ * max_samples rows (0 stops collecting and frees the buffer). */
+int mqe_collect_samples(mqe_engine *e, long max_samples);
+int mqe_num_samples(mqe_engine *e, long *n);
+int mqe_read_samples(mqe_engine *e, long first, long n, float *X, float *Y);
+/* mqe_mlp_fit on the collected rows, which stay on the device */
+int mqe_train_proposal(mqe_engine *e, mqe_mlp *m, int epochs, int batch, double lr, double *loss);
+/* Sample the following propagations from the network's proposal (m = NULL:
+ * the reference's proposal).  The network must have n_in = MQE_NFEAT and
+ * n_out = MQE_NLABEL and stay alive while it is in use. */
+int mqe_set_proposal(mqe_engine *e, mqe_mlp *m, double mix);
+/* Fixed predictions instead of a network: n_walkers * nC * 16 floats in walker
+ * order (NULL: the reference's proposal). */
+int mqe_set_proposal_table(mqe_engine *e, const float *pred, int n_walkers, double mix);
+/* Local energies of the bonds (couplings) from the last energy sweep, n * nC
+ * doubles in walker order; they sum to each walker's E_L. */
+int mqe_read_bond_energies(mqe_engine *e, double *out);
+
 /* ---- set-up helpers (host code; no GPU needed) ----------------------------- */
 
 /* Bond dimensions of MPSstate(L, D, d) (MPSstate.cpp:20-28). */

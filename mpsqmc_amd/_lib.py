@@ -75,6 +75,27 @@ SIGNATURES = {
     "mqe_set_timing": (c_int, [ctypes.c_void_p, c_int]),
     "mqe_kernel_time": (c_int, [ctypes.c_void_p, P(c_double), P(c_int)]),
     "mqe_last_error": (ctypes.c_char_p, []),
+    "mqe_mlp_create": (c_int, [c_int, c_int, c_int, ctypes.c_ulonglong, P(ctypes.c_void_p)]),
+    "mqe_mlp_destroy": (None, [ctypes.c_void_p]),
+    "mqe_mlp_num_params": (c_long, [ctypes.c_void_p]),
+    "mqe_mlp_get_params": (c_int, [ctypes.c_void_p, P(ctypes.c_float)]),
+    "mqe_mlp_set_params": (c_int, [ctypes.c_void_p, P(ctypes.c_float)]),
+    "mqe_mlp_get_scaling": (c_int, [ctypes.c_void_p, P(ctypes.c_float), P(ctypes.c_float), P(ctypes.c_float),
+                                    P(ctypes.c_float)]),
+    "mqe_mlp_set_scaling": (c_int, [ctypes.c_void_p, P(ctypes.c_float), P(ctypes.c_float), P(ctypes.c_float),
+                                    P(ctypes.c_float)]),
+    "mqe_mlp_forward": (c_int, [ctypes.c_void_p, c_int, P(ctypes.c_float), P(ctypes.c_float)]),
+    "mqe_mlp_loss_grad": (c_int, [ctypes.c_void_p, c_int, P(ctypes.c_float), P(ctypes.c_float), P(c_double),
+                                  P(ctypes.c_float)]),
+    "mqe_mlp_fit": (c_int, [ctypes.c_void_p, c_int, P(ctypes.c_float), P(ctypes.c_float), c_int, c_int, c_double,
+                            c_int, P(c_double)]),
+    "mqe_collect_samples": (c_int, [ctypes.c_void_p, c_long]),
+    "mqe_num_samples": (c_int, [ctypes.c_void_p, P(c_long)]),
+    "mqe_read_samples": (c_int, [ctypes.c_void_p, c_long, c_long, P(ctypes.c_float), P(ctypes.c_float)]),
+    "mqe_train_proposal": (c_int, [ctypes.c_void_p, ctypes.c_void_p, c_int, c_int, c_double, P(c_double)]),
+    "mqe_set_proposal": (c_int, [ctypes.c_void_p, ctypes.c_void_p, c_double]),
+    "mqe_set_proposal_table": (c_int, [ctypes.c_void_p, P(ctypes.c_float), c_int, c_double]),
+    "mqe_read_bond_energies": (c_int, [ctypes.c_void_p, P(c_double)]),
     "mqe_save_state": (c_int, [ctypes.c_void_p, ctypes.c_char_p, c_long]),
     "mqe_load_state": (c_int, [ctypes.c_void_p, ctypes.c_char_p, P(c_long)]),
     "mqe_bond_dims": (None, [c_int, c_int, c_int, P(c_int)]),

@@ -81,6 +81,14 @@ struct Pool {
     char *scratch;
     long scratch_bytes;
     int padded;            // 1: tile32 storage (32x32 zero-padded blocks per site and physical index)
+    // learned proposal (tile path)
+    double *bondE;         // per slot: nC bond energies <trial|h_c|phi> and <trial|phi> (or nullptr)
+    int nbe;               // entries of bondE per slot (nC + 1)
+    const float *pred;     // predicted labels, row b*nC + c, 16 each (or nullptr)
+    double mix;            // weight of the uniform part of the learned proposal
+    float *lab;            // label rows of the sample buffer (or nullptr)
+    const long *nsamp;     // rows already in the sample buffer
+    long maxsamp;          // its capacity in rows
 };
 
 // Scratch layout of one workgroup (sizes in elements).
@@ -947,6 +955,42 @@ __global__ __launch_bounds__(MQE_NT) void copy_walkers_kernel(Book B, Pool P, in
         P.eh[dst] = P.eh[src];
         P.EL[dst] = P.EL[src];
     }
+    if (P.bondE)
+        for (int k = threadIdx.x; k < P.nbe; k += MQE_NT)
+            P.bondE[(size_t)dst * P.nbe + k] = P.bondE[(size_t)src * P.nbe + k];
+}
+
+// Features of the learned proposal, row b*nC + c (MQE_NFEAT floats): log O,
+// log w, E_L, e_{c-1}, e_c, e_{c+1}, c/(nC-1) with e_k = bond energy / overlap
+// of the last energy sweep.  X: rows of this step (or nullptr); the sample
+// buffer gets the same rows after its *nsamp rows if they fit.
+__global__ void feature_kernel(Pool P, int nC, float *X, float *ring) {
+    const int n = *P.n;
+    const long base = P.nsamp ? *P.nsamp : 0;
+    const bool to_ring = ring && base + (long)n * nC <= P.maxsamp;
+    for (int b = blockIdx.x; b < n; b += gridDim.x) {
+        const int slot = P.order[b];
+        const double *be = P.bondE + (size_t)slot * (nC + 1);
+        const double inv = be[nC] != 0.0 ? 1.0 / be[nC] : 0.0;  // no sweep yet: zero bond features
+        const float lO = (float)log(P.O[slot]), lw = (float)log(fmax(P.w[slot], 1e-300)), EL = (float)P.EL[slot];
+        for (int c = threadIdx.x; c < nC; c += blockDim.x) {
+            float f[MQE_NFEAT] = {lO, lw, EL, c > 0 ? (float)(be[c - 1] * inv) : 0.f, (float)(be[c] * inv),
+                                  c + 1 < nC ? (float)(be[c + 1] * inv) : 0.f, nC > 1 ? (float)c / (nC - 1) : 0.f};
+            const long row = (long)b * nC + c;
+            for (int q = 0; q < MQE_NFEAT; ++q) {
+                if (X) X[row * MQE_NFEAT + q] = f[q];
+                if (to_ring) ring[(base + row) * MQE_NFEAT + q] = f[q];
+            }
+        }
+    }
+}
+
+// The sample buffer took this step's rows (if they fitted).
+__global__ void advance_samples_kernel(const int *n, int nC, long *nsamp, long maxsamp) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const long add = (long)(*n) * nC;
+        if (*nsamp + add <= maxsamp) *nsamp += add;
+    }
 }
 
 __global__ void install_order_kernel(Book B) {
@@ -1005,6 +1049,15 @@ struct mqe_engine {
     long u_cap = 0;
     char *d_scratch = nullptr;
     int *d_status = nullptr;
+    // learned proposal (tile path)
+    double *d_bond = nullptr;                  // per slot: nC bond energies and the overlap of the last sweep
+    float *d_feat = nullptr, *d_pred = nullptr;  // rows b*nC + c of the current step
+    float *d_sX = nullptr, *d_sY = nullptr;    // collected samples (features, labels)
+    long *d_nsamp = nullptr;
+    long max_samples = 0;
+    mqe_mlp *net = nullptr;
+    bool learn = false;
+    double mix = 0.1;
     double *d_sums = nullptr, *d_trace = nullptr;
     int trace_cap = 0;
     int *d_ncopy = nullptr, *d_newpos = nullptr, *d_used = nullptr, *d_free = nullptr, *d_csrc = nullptr,
@@ -1049,6 +1102,13 @@ static Pool make_pool(mqe_engine *e, const double *u, int step) {
     P.scratch = e->d_scratch;
     P.scratch_bytes = e->scratch_bytes;
     P.padded = e->tile32 ? 1 : 0;
+    P.bondE = e->d_bond;
+    P.nbe = e->nC + 1;
+    P.pred = e->learn ? e->d_pred : nullptr;
+    P.mix = e->mix;
+    P.lab = e->max_samples > 0 ? e->d_sY : nullptr;
+    P.nsamp = e->d_nsamp;
+    P.maxsamp = e->max_samples;
     return P;
 }
 
@@ -1093,8 +1153,30 @@ static int launch_walker_any(mqe_engine *e, const double *u, int step, int flags
     }
     if (e->tile32) {
         Pool P = make_pool(e, u, step);
-        hipLaunchKernelGGL(t32::walker32_kernel, dim3(e->p.capacity), dim3(MQE_NT), t32::lds_bytes(), e->stream,
-                           e->sys, P, (const float *)e->d_t32N, (const float *)e->d_t32T, flags);
+        const bool prop = (flags & F_PROP) != 0;
+        const bool collect = prop && e->max_samples > 0, learn = prop && e->learn;
+        if (collect || (learn && e->net)) {
+            hipLaunchKernelGGL(feature_kernel, dim3(std::min(e->p.capacity, 4096)), dim3(64), 0, e->stream, P, e->nC,
+                               learn && e->net ? e->d_feat : nullptr, collect ? e->d_sX : nullptr);
+            HIPCHK(hipGetLastError());
+        }
+        if (learn && e->net) {
+            const int r = mlp_forward_device(e->net, e->d_feat, (long)e->p.capacity * e->nC, e->d_n, e->nC,
+                                             e->d_pred, e->stream);
+            if (r != MQE_OK) return r;
+        }
+        if (!collect) P.lab = nullptr;
+        if (learn)
+            hipLaunchKernelGGL(t32::walker32_kernel<true>, dim3(e->p.capacity), dim3(MQE_NT), t32::lds_bytes(),
+                               e->stream, e->sys, P, (const float *)e->d_t32N, (const float *)e->d_t32T, flags);
+        else
+            hipLaunchKernelGGL(t32::walker32_kernel<false>, dim3(e->p.capacity), dim3(MQE_NT), t32::lds_bytes(),
+                               e->stream, e->sys, P, (const float *)e->d_t32N, (const float *)e->d_t32T, flags);
+        HIPCHK(hipGetLastError());
+        if (collect) {
+            hipLaunchKernelGGL(advance_samples_kernel, dim3(1), dim3(64), 0, e->stream, e->d_n, e->nC, e->d_nsamp,
+                               e->max_samples);
+        }
     } else if (e->fp32) launch_walker<float>(e, u, step, flags);
     else launch_walker<double>(e, u, step, flags);
     HIPCHK(hipGetLastError());
@@ -1249,8 +1331,14 @@ int mqe_create(const mqe_params *p, const mqe_mps *trial, int n_couplings, const
         CK(hipMalloc((void **)&e->d_t32T, sizeof(float) * tt.size()));
         CK(hipMemcpy(e->d_t32N, tn.data(), sizeof(float) * tn.size(), hipMemcpyHostToDevice));
         CK(hipMemcpy(e->d_t32T, tt.data(), sizeof(float) * tt.size(), hipMemcpyHostToDevice));
-        CK(hipFuncSetAttribute((const void *)t32::walker32_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+        CK(hipFuncSetAttribute((const void *)t32::walker32_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)t32::lds_bytes()));
+        CK(hipFuncSetAttribute((const void *)t32::walker32_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)t32::lds_bytes()));
+        CK(hipMalloc((void **)&e->d_bond, sizeof(double) * (size_t)p->capacity * (n_couplings + 1)));
+        CK(hipMemset(e->d_bond, 0, sizeof(double) * (size_t)p->capacity * (n_couplings + 1)));
+        CK(hipMalloc((void **)&e->d_nsamp, sizeof(long)));
+        CK(hipMemset(e->d_nsamp, 0, sizeof(long)));
     }
     const int nC = n_couplings;
     std::vector<int> nextfirst(std::max(nC, 1)), csptr(L + 2, 0), csidx(std::max(nC, 1));
@@ -1371,7 +1459,7 @@ void mqe_destroy(mqe_engine *e) {
                     e->d_Renv, e->d_Lenv, e->d_wvd, e->d_w, e->d_O, e->d_eh, e->d_EL, e->d_order, e->d_order2,
                     e->d_n, e->d_idx, e->d_u, e->d_scratch, e->d_status, e->d_sums, e->d_trace, e->d_ncopy,
                     e->d_newpos, e->d_used, e->d_free, e->d_csrc, e->d_cdst, e->d_nextra, e->d_nnew, e->d_nused,
-                    e->d_tmpout};
+                    e->d_tmpout, e->d_bond, e->d_feat, e->d_pred, e->d_sX, e->d_sY, e->d_nsamp};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     for (hipEvent_t ev : e->tev) (void)hipEventDestroy(ev);
@@ -1759,6 +1847,131 @@ struct StateHeader {
     int n, prepared;
     long prop_count, step;
 };
+
+// ---- learned proposal -------------------------------------------------------
+
+static int need_tile(mqe_engine *e, const char *who) {
+    if (!e) { set_error((std::string(who) + ": null engine").c_str()); return MQE_EINVAL; }
+    if (!e->tile32) {
+        set_error((std::string(who) + ": the learned proposal runs on the tile path only "
+                   "(nearest-neighbour chain, bonds <= 32, FP32, h = 0)").c_str());
+        return MQE_EINVAL;
+    }
+    return MQE_OK;
+}
+
+int mqe_collect_samples(mqe_engine *e, long max_samples) {
+    ECHK(need_tile(e, "mqe_collect_samples"));
+    if (max_samples < 0) { set_error("mqe_collect_samples: negative size"); return MQE_EINVAL; }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (e->d_sX) HIPCHK(hipFree(e->d_sX));
+    if (e->d_sY) HIPCHK(hipFree(e->d_sY));
+    e->d_sX = e->d_sY = nullptr;
+    e->max_samples = 0;
+    HIPCHK(hipMemset(e->d_nsamp, 0, sizeof(long)));
+    if (max_samples == 0) return MQE_OK;
+    HIPCHK(hipMalloc((void **)&e->d_sX, sizeof(float) * (size_t)max_samples * MQE_NFEAT));
+    HIPCHK(hipMalloc((void **)&e->d_sY, sizeof(float) * (size_t)max_samples * MQE_NLABEL));
+    e->max_samples = max_samples;
+    return MQE_OK;
+}
+
+int mqe_num_samples(mqe_engine *e, long *n) {
+    ECHK(need_tile(e, "mqe_num_samples"));
+    if (!n) { set_error("mqe_num_samples: null argument"); return MQE_EINVAL; }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    HIPCHK(hipMemcpy(n, e->d_nsamp, sizeof(long), hipMemcpyDeviceToHost));
+    return MQE_OK;
+}
+
+int mqe_read_samples(mqe_engine *e, long first, long n, float *X, float *Y) {
+    long have = 0;
+    ECHK(mqe_num_samples(e, &have));
+    if (first < 0 || n < 0 || first + n > have) { set_error("mqe_read_samples: rows out of range"); return MQE_EINVAL; }
+    if (X) HIPCHK(hipMemcpy(X, e->d_sX + first * MQE_NFEAT, sizeof(float) * n * MQE_NFEAT, hipMemcpyDeviceToHost));
+    if (Y) HIPCHK(hipMemcpy(Y, e->d_sY + first * MQE_NLABEL, sizeof(float) * n * MQE_NLABEL, hipMemcpyDeviceToHost));
+    return MQE_OK;
+}
+
+static int check_net(mqe_mlp *m, const char *who) {
+    int nin = 0, nout = 0;
+    mlp_io_dims(m, &nin, &nout);
+    if (nin != MQE_NFEAT || nout != MQE_NLABEL) {
+        set_error((std::string(who) + ": the network must map MQE_NFEAT features to MQE_NLABEL labels").c_str());
+        return MQE_EINVAL;
+    }
+    return MQE_OK;
+}
+
+int mqe_train_proposal(mqe_engine *e, mqe_mlp *m, int epochs, int batch, double lr, double *loss) {
+    ECHK(need_tile(e, "mqe_train_proposal"));
+    if (!m) { set_error("mqe_train_proposal: null network"); return MQE_EINVAL; }
+    ECHK(check_net(m, "mqe_train_proposal"));
+    long n = 0;
+    ECHK(mqe_num_samples(e, &n));
+    if (n < 1) { set_error("mqe_train_proposal: no samples collected"); return MQE_ESTATE; }
+    return mlp_fit_device(m, e->d_sX, e->d_sY, n, epochs, batch, lr, 1, loss, e->stream);
+}
+
+static int ensure_pred(mqe_engine *e) {
+    if (e->d_pred) return MQE_OK;
+    const size_t rows = (size_t)e->p.capacity * e->nC;
+    HIPCHK(hipMalloc((void **)&e->d_feat, sizeof(float) * rows * MQE_NFEAT));
+    HIPCHK(hipMalloc((void **)&e->d_pred, sizeof(float) * rows * MQE_NLABEL));
+    return MQE_OK;
+}
+
+int mqe_set_proposal(mqe_engine *e, mqe_mlp *m, double mix) {
+    ECHK(need_tile(e, "mqe_set_proposal"));
+    if (!(mix >= 0.0 && mix <= 1.0)) { set_error("mqe_set_proposal: mix must lie in [0, 1]"); return MQE_EINVAL; }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (!m) {
+        e->learn = false;
+        e->net = nullptr;
+        return MQE_OK;
+    }
+    ECHK(check_net(m, "mqe_set_proposal"));
+    ECHK(ensure_pred(e));
+    e->net = m;
+    e->learn = true;
+    e->mix = mix;
+    return MQE_OK;
+}
+
+int mqe_set_proposal_table(mqe_engine *e, const float *pred, int n_walkers, double mix) {
+    ECHK(need_tile(e, "mqe_set_proposal_table"));
+    if (!(mix >= 0.0 && mix <= 1.0)) { set_error("mqe_set_proposal_table: mix must lie in [0, 1]"); return MQE_EINVAL; }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    e->net = nullptr;
+    if (!pred) {
+        e->learn = false;
+        return MQE_OK;
+    }
+    if (n_walkers < 0 || n_walkers > e->p.capacity) { set_error("mqe_set_proposal_table: bad walker count"); return MQE_EINVAL; }
+    ECHK(ensure_pred(e));
+    HIPCHK(hipMemcpy(e->d_pred, pred, sizeof(float) * (size_t)n_walkers * e->nC * MQE_NLABEL, hipMemcpyHostToDevice));
+    e->learn = true;
+    e->mix = mix;
+    return MQE_OK;
+}
+
+int mqe_read_bond_energies(mqe_engine *e, double *out) {
+    ECHK(need_tile(e, "mqe_read_bond_energies"));
+    if (!out) { set_error("mqe_read_bond_energies: null argument"); return MQE_EINVAL; }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    HIPCHK(hipMemcpy(&e->n_host, e->d_n, sizeof(int), hipMemcpyDeviceToHost));
+    const int n = e->n_host, cap = e->p.capacity, nb = e->nC + 1;
+    std::vector<int> ord(n);
+    HIPCHK(hipMemcpy(ord.data(), e->d_order, sizeof(int) * n, hipMemcpyDeviceToHost));
+    std::vector<double> tmp((size_t)cap * nb);
+    HIPCHK(hipMemcpy(tmp.data(), e->d_bond, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i) {
+        const double *be = tmp.data() + (size_t)ord[i] * nb;
+        const double inv = be[e->nC] != 0.0 ? 1.0 / be[e->nC] : 0.0;
+        for (int c = 0; c < e->nC; ++c) out[(size_t)i * e->nC + c] = be[c] * inv;
+    }
+    return MQE_OK;
+}
 
 int mqe_save_state(mqe_engine *e, const char *path, long step) {
     if (!e || !path) { set_error("mqe_save_state: bad arguments"); return MQE_EINVAL; }

@@ -239,6 +239,7 @@ struct W32 {
     float *sm;              // LDS floats
     const int *wv;          // walker bond dims (LDS)
     double *red;            // LDS reduction scratch
+    double *bond;           // this walker's bond energies (nC raw values, then the overlap) or nullptr
     __device__ float *blk(int b) const { return sm + (size_t)b * BLK; }
     __device__ const float *tn(int k, int s) const { return TN + (size_t)(2 * k + s) * GB; }
     __device__ const float *tt(int k, int s) const { return TT + (size_t)(2 * k + s) * GB; }
@@ -521,6 +522,7 @@ __device__ double energy_sweep32(const W32 &C, int pc, double *On, double *Ou, b
             double e = 0.0;
             for (int q = 0; q < 16; ++q) e += S.SS[q] * Mq[q];
             etot += S.Jc[k - 1] * e;
+            if (C.bond && threadIdx.x == 0) C.bond[k - 1] = S.Jc[k - 1] * e;
             int t = pc; pc = pj; pj = t;
         }
         double ov, n2;
@@ -533,6 +535,7 @@ __device__ double energy_sweep32(const W32 &C, int pc, double *On, double *Ou, b
         __syncthreads();
         l2g(c0, C.ph(0, 0));
         l2g(c0 + BLK, C.ph(0, 1));
+        if (C.bond && threadIdx.x == 0) C.bond[S.nC] = ov;
         *On = O;
         *Ou = ov;
         return etot;
@@ -595,6 +598,7 @@ __device__ double energy_sweep32(const W32 &C, int pc, double *On, double *Ou, b
         double e = 0.0;
         for (int q = 0; q < 16; ++q) e += S.SS[q] * Mq[q];
         etot += S.Jc[k - 1] * e;
+        if (C.bond && threadIdx.x == 0) C.bond[k - 1] = S.Jc[k - 1] * e;
         int t = pc; pc = pj; pj = t;
         __syncthreads();
     }
@@ -608,11 +612,14 @@ __device__ double energy_sweep32(const W32 &C, int pc, double *On, double *Ou, b
     __syncthreads();
     l2g(c0, C.ph(0, 0));
     l2g(c0 + BLK, C.ph(0, 1));
+    if (C.bond && threadIdx.x == 0) C.bond[S.nC] = ov;
     *On = O;
     *Ou = ov;
     return etot;
 }
 
+// LEARN: sample from the learned proposal P.pred with the importance weight
+template <bool LEARN>
 __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, const float *TN, const float *TT,
                                                               int flags) {
     extern __shared__ float lds32[];
@@ -637,6 +644,7 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
     C.sm = lds32;
     C.wv = wv;
     C.red = red;
+    C.bond = P.bondE ? P.bondE + (size_t)slot * (S.nC + 1) : nullptr;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     double wt = P.w[slot], O = P.O[slot];
 
@@ -706,6 +714,8 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
     for (int c = 0; c < S.nC; ++c) {
         const int i = c, j = c + 1;
         float *cen = C.blk(pc), *nxt = C.blk(pj);
+        float plab = 0.f;  // learned proposal: predicted label lane of this coupling (lanes < 16)
+        if constexpr (LEARN) plab = lane < 16 ? P.pred[((size_t)b * S.nC + c) * 16 + lane] : 0.f;
         // trial operands of stage AB: T_i[s']^T (waves 0,1) or T_j[t'] (waves 2,3), s', t' = 0, 1
         frag16 f0, f1;
         if (w < 2) {
@@ -806,6 +816,10 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
 #pragma unroll
         for (int q = 0; q < 16; ++q) cb[q] = readlane_d(cq, q);
         const double pref = 4.0 / (O * Ng);
+        if (P.lab && w == 0 && lane < 16) {  // training label combos/O
+            const long base = *P.nsamp, row = base + (long)b * S.nC + c;
+            if (base + (long)n * S.nC <= P.maxsamp) P.lab[row * 16 + lane] = (float)(cq / O);
+        }
         double pv[4], loc = 0.0;
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
@@ -842,8 +856,58 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
             run += pv[t];
             if (4 * lane + t < Ng - 1 && run <= u) cnt += 1.0;
         }
-        const int gs = (int)wave_sum(cnt);
-        wt *= pdfnorm;
+        int gs = (int)wave_sum(cnt);
+        if constexpr (LEARN) {
+            // learned proposal q from the predicted labels; weight pdf(x)/q(x)
+            double ph[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) ph[q] = (double)__int_as_float(__builtin_amdgcn_readlane(__float_as_int(plab), q));
+            auto quad = [&](const double *cf, const double *x) {
+                double f = 0.0;
+#pragma unroll
+                for (int l = 0; l < 4; ++l) {
+                    double t = 0.0;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) t += cf[l + 4 * r] * x[r];
+                    f += x[l] * t;
+                }
+                return f;
+            };
+            double qv[4], qloc = 0.0;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int gq = 4 * lane + t;
+                const double v = gq < Ng ? fmax(quad(ph, grid_s + 4 * gq), 0.0) : 0.0;
+                qv[t] = v;
+                qloc += v;
+            }
+            const double qs = wave_sum(qloc);
+            const double a1 = qs > 0.0 ? (1.0 - P.mix) / qs : 0.0, a0 = qs > 0.0 ? P.mix / Ng : 1.0 / Ng;
+            double ql = 0.0;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                qv[t] = 4 * lane + t < Ng ? a1 * qv[t] + a0 : 0.0;
+                ql += qv[t];
+            }
+            double qrun = wave_exclusive_scan(ql), qcnt = 0.0;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                qrun += qv[t];
+                if (4 * lane + t < Ng - 1 && qrun <= u) qcnt += 1.0;
+            }
+            const int gl = (int)wave_sum(qcnt);
+            const double *xl = grid_s + 4 * gl;
+            const double vl = fmax(quad(cb, xl), 0.0) * pref;
+            const double qgl = a1 * fmax(quad(ph, xl), 0.0) + a0;
+            if (vl > 0.0) {
+                gs = gl;
+                wt *= vl / qgl;
+            } else {
+                wt = 0.0;  // continues with the reference's own sample
+            }
+        } else {
+            wt *= pdfnorm;
+        }
         if (threadIdx.x == 0) {
             if (P.idx) P.idx[(size_t)b * S.nC + c] = gs;
             if (!(pdfnorm > 0.0) || !isfinite(pdfnorm)) atomicOr(P.status, 1);

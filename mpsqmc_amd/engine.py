@@ -129,6 +129,53 @@ class Engine:
     def set_canonical_interval(self, steps):
         check(lib().mqe_set_canonical_interval(self.h, steps))
 
+    # -- learned proposal (tile path) ------------------------------------------
+    def collect_samples(self, max_samples):
+        """Store (features, combos/O) rows of the following propagations, up to max_samples."""
+        check(lib().mqe_collect_samples(self.h, int(max_samples)))
+
+    def num_samples(self):
+        n = ctypes.c_long()
+        check(lib().mqe_num_samples(self.h, ctypes.byref(n)))
+        return n.value
+
+    def read_samples(self, first=0, n=None):
+        from .mlp import NFEAT, NLABEL
+        if n is None:
+            n = self.num_samples() - first
+        X = np.zeros((n, NFEAT), np.float32)
+        Y = np.zeros((n, NLABEL), np.float32)
+        fp = ctypes.POINTER(ctypes.c_float)
+        check(lib().mqe_read_samples(self.h, int(first), int(n), X.ctypes.data_as(fp), Y.ctypes.data_as(fp)))
+        return X, Y
+
+    def train_proposal(self, net, epochs=5, batch=1024, lr=1e-3):
+        """Fit the network on the collected rows (on the device); returns the last epoch's loss."""
+        loss = ctypes.c_double()
+        check(lib().mqe_train_proposal(self.h, net.h, int(epochs), int(batch), float(lr), ctypes.byref(loss)))
+        return loss.value
+
+    def set_proposal(self, net, mix=0.1):
+        """Sample from the network's proposal with the importance weight (None: the reference's)."""
+        self._net = net
+        check(lib().mqe_set_proposal(self.h, net.h if net is not None else None, float(mix)))
+
+    def set_proposal_table(self, pred, mix=0.1):
+        """Fixed predicted labels (n x nC x 16, walker order) instead of a network; None clears."""
+        if pred is None:
+            check(lib().mqe_set_proposal_table(self.h, None, 0, 0.0))
+            return
+        pred = np.ascontiguousarray(pred, dtype=np.float32).reshape(-1, self.nC, 16)
+        fp = ctypes.POINTER(ctypes.c_float)
+        check(lib().mqe_set_proposal_table(self.h, pred.ctypes.data_as(fp), pred.shape[0], float(mix)))
+
+    def bond_energies(self):
+        """Bond local energies of the last energy sweep (n x nC); rows sum to E_L."""
+        n = self.num_walkers()
+        out = np.zeros((n, self.nC))
+        check(lib().mqe_read_bond_energies(self.h, _dptr(out)))
+        return out
+
     # -- checkpoint -----------------------------------------------------------
     def save_state(self, path, step=0):
         """Write the population snapshot (mqe_save_state) with a step number."""

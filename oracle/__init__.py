@@ -64,6 +64,8 @@ def lib():
         L.orc_pop_get_trial.argtypes = [ctypes.c_void_p, _ip, _dp]
         L.orc_pop_get_htrial.restype = ctypes.c_long
         L.orc_pop_get_htrial.argtypes = [ctypes.c_void_p, _ip, _dp]
+        L.orc_pop_set_proposal.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double]
+        L.orc_pop_set_labels.argtypes = [ctypes.c_void_p, _dp]
         L.orc_pop_propagate.restype = ctypes.c_double
         L.orc_pop_propagate.argtypes = [ctypes.c_void_p, _dp, _ip]
         L.orc_pop_energy.restype = ctypes.c_double
@@ -307,6 +309,28 @@ class Population:
         idx = np.zeros(self.nwalkers * self.nC, dtype=np.int32)
         s = lib().orc_pop_propagate(self.h, u.ctypes.data_as(_dp), idx.ctypes.data_as(_ip))
         return s, idx.reshape(self.nwalkers, self.nC)
+
+    def set_proposal(self, pred, mix=0.1):
+        """Sample from the proposal built from predicted labels (nwalkers x nC x 16,
+        walker order) with the importance weight; None restores the reference's."""
+        if pred is None:
+            self._pred = None
+            lib().orc_pop_set_proposal(self.h, None, 0.0)
+            return
+        self._pred = np.ascontiguousarray(pred, dtype=np.float64).reshape(-1)
+        if self._pred.size != self.nwalkers * self.nC * 16:
+            raise ValueError("pred must hold nwalkers * nC * 16 values")
+        lib().orc_pop_set_proposal(self.h, self._pred.ctypes.data_as(_dp), float(mix))
+
+    def propagate_with_labels(self, u):
+        """propagate(u) that also returns combos/O per (walker, coupling)."""
+        lab = np.zeros((self.nwalkers, self.nC, 16))
+        lib().orc_pop_set_labels(self.h, lab.ctypes.data_as(_dp))
+        try:
+            s, idx = self.propagate(u)
+        finally:
+            lib().orc_pop_set_labels(self.h, None)
+        return s, idx, lab
 
     def energy(self, step, dofluct=True):
         EP = ctypes.c_double()

@@ -957,7 +957,17 @@ typedef struct {
                                2: <H trial|phi> with H|trial> from the minimal-channel MPO */
     omps hmin;
     double *Jfull;
+    /* learned proposal (the engine's extension; no counterpart in the reference) */
+    const double *pred;     /* predicted labels per (walker, coupling), 16 each, or NULL */
+    double mix;
+    double *lab;            /* receives combos/O per (walker, coupling), or NULL */
 } orc_pop;
+
+/* Proposal from predicted labels (the engine's learned proposal, restated):
+   q(x) = (1-mix) qhat(x)/sum qhat + mix/N with qhat(x) = max(0, sum p x x),
+   uniform when every qhat is 0. */
+ORC_API void orc_pop_set_proposal(orc_pop *p, const double *pred, double mix) { p->pred = pred; p->mix = mix; }
+ORC_API void orc_pop_set_labels(orc_pop *p, double *lab) { p->lab = lab; }
 
 ORC_API int orc_nthreads(void) {
 #ifdef _OPENMP
@@ -1203,13 +1213,48 @@ static void propagate_walker(orc_pop *p, int i, const double *u /* nC uniforms *
             pdf[g] = v;
             norm += v;
         }
+        if (p->lab)
+            for (int q = 0; q < n2 * n2; q++) p->lab[((long)i * p->nC + c) * n2 * n2 + q] = combos[q] / p->ov[i];
+        double v_raw = 0.0; /* unnormalised PDF value of the learned sample */
+        int lcnt = -1;
+        double rn = u[c];
+        if (p->pred) {
+            const double *ph = p->pred + ((long)i * p->nC + c) * n2 * n2;
+            double qs = 0.0;
+            for (int g = 0; g < N; g++) {
+                double v = 0.0;
+                for (int l = 0; l < n2; l++)
+                    for (int r = 0; r < n2; r++) v += ph[l + n2 * r] * p->grid[l + dim * g] * p->grid[r + dim * g];
+                qs += v > 0.0 ? v : 0.0;
+            }
+            double a1 = qs > 0.0 ? (1.0 - p->mix) / qs : 0.0, a0 = qs > 0.0 ? p->mix / N : 1.0 / N;
+            double qx;
+            int cnt = 0;
+            double cum = 0.0;
+            for (int g = 0; g < N; g++) {
+                double v = 0.0;
+                for (int l = 0; l < n2; l++)
+                    for (int r = 0; r < n2; r++) v += ph[l + n2 * r] * p->grid[l + dim * g] * p->grid[r + dim * g];
+                cum += a1 * (v > 0.0 ? v : 0.0) + a0;
+                if (g < N - 1 && cum <= rn) cnt++;
+            }
+            {
+                double v = 0.0;
+                for (int l = 0; l < n2; l++)
+                    for (int r = 0; r < n2; r++) v += ph[l + n2 * r] * p->grid[l + dim * cnt] * p->grid[r + dim * cnt];
+                qx = a1 * (v > 0.0 ? v : 0.0) + a0;
+            }
+            v_raw = pdf[cnt];
+            if (v_raw > 0.0) { lcnt = cnt; p->wt[i] *= v_raw / qx; }
+            else p->wt[i] = 0.0;
+        }
         double inv = 1.0 / norm;
         for (int g = 0; g < N; g++) pdf[g] *= inv;
-        p->wt[i] *= norm;
-        double rn = u[c];
+        if (!p->pred) p->wt[i] *= norm;
         int cnt = 0;
         double cum = pdf[0];
         while (cum <= rn && cnt < N - 1) { cnt++; cum += pdf[cnt]; }
+        if (lcnt >= 0) cnt = lcnt;
         if (idx) idx[c] = cnt;
         apply_grid_term(p, m, c, cnt);
         p->ov[i] = set_overlap(m, &p->trial);
