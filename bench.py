@@ -52,6 +52,11 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=20131015)
     ap.add_argument("--random-trial", action="store_true", help="random trial instead of the DMRG one")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal comparison")
+    ap.add_argument("--learned-samples", type=int, default=1_000_000)
+    ap.add_argument("--learned-epochs", type=int, default=4)
+    ap.add_argument("--learned-mix", type=float, default=0.01,
+                    help="weight of the uniform part of the learned proposal (keeps its support)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB ten times (reference for HBM counters)")
     return ap.parse_args()
@@ -118,6 +123,44 @@ def cpu_baseline(system, vd, data, walkers, threads):
     return dict(value=walkers / dt, unit="walker-steps/s", cores=threads, kind="port",
                 sample=f"{walkers} walkers x 1 step of the L={system.L}, D={int(vd.max())} chain "
                        f"(reference algorithm, FP64, {threads} OpenMP threads), {dt:.1f} s")
+
+
+def learned_comparison(eng, args, first, plain_vt):
+    """The learned proposal on the same walk: collect (feature, exact label)
+    rows from plain steps, fit the MLP on the GPU, then time as many steps with
+    the learned proposal and compare variance x GPU-s with the plain run."""
+    from mpsqmc_amd.analysis import variance_time
+    from mpsqmc_amd.mlp import MLP
+    rows_per_step = args.walkers * eng.nC
+    n_collect = -(-args.learned_samples // rows_per_step) + 1
+    eng.set_timing(False)
+    eng.collect_samples(args.learned_samples)
+    eng.run(first, n_collect)
+    eng.sync()
+    n_rows = eng.num_samples()
+    net = MLP(seed=args.seed)
+    t0 = time.perf_counter()
+    loss = eng.train_proposal(net, epochs=args.learned_epochs, batch=4096, lr=1e-3)
+    eng.sync()
+    t_train = time.perf_counter() - t0
+    log(f"bench: learned proposal trained on {n_rows} rows in {t_train:.2f} s (loss {loss:.4g})")
+    eng.collect_samples(0)
+    eng.set_proposal(net, mix=args.learned_mix)
+    first2 = first + n_collect
+    t0 = time.perf_counter()
+    eng.run(first2, args.steps)
+    eng.sync()
+    wall = time.perf_counter() - t0
+    tr = eng.read_trace(first2, args.steps)
+    eng.set_proposal(None)
+    net.close()
+    vt = variance_time(tr[:, 2], wall)
+    return {"samples": int(n_rows), "epochs": args.learned_epochs, "train_loss": loss, "train_s": t_train,
+            "mix": args.learned_mix, "value": float(tr[:, 1].sum()) / wall, "unit": "walker-steps/s",
+            "population_mean": float(tr[:, 1].mean()),
+            "variance_x_gpu_s": vt["variance_x_seconds"], "E_mean": vt["mean"], "E_error": vt["error"],
+            "block_length": vt["block_length"], "steps": args.steps, "seconds": wall,
+            "gain_vs_plain": (plain_vt["variance_x_seconds"] / vt["variance_x_seconds"]) if plain_vt else None}
 
 
 def main():
@@ -198,8 +241,11 @@ def main():
             "steps": vt["n"], "seconds": vt["seconds"]},
         "E_trial": E_trial,
         "E_P_last": float(trace[-1, 2]),
+        "population_mean": float(trace[:, 1].mean()),
         "population_last": int(trace[-1, 1]),
     }
+    if not args.no_learned and vt is not None and eng.path == "tile32":
+        out["learned_proposal"] = learned_comparison(eng, args, first + args.steps, vt)
     eng.close()
     if not args.no_cpu:
         log(f"bench: CPU baseline on {args.cpu_walkers} walkers")
