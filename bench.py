@@ -79,15 +79,16 @@ TRAFFIC_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "prof
                                "walker32_kernel_chain_L32_D32_N2048_hbm.md")
 
 
-def measured_traffic(L, D, N):
-    """HBM bytes per walker-kernel launch from the committed counter summary
-    (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel reference),
-    for the workload it was measured on; None otherwise."""
+def measured_traffic(L, D, N, walkers_per_launch):
+    """HBM bytes per walker-kernel launch: the bytes per walker-step measured
+    with rocprofv3 FETCH_SIZE + WRITE_SIZE (scaled by the copy-kernel
+    reference) in the committed counter summary, times the walkers of an
+    average launch of this run; None for other workloads."""
     if (L, D, N) != (32, 32, 2048) or not os.path.exists(TRAFFIC_SUMMARY):
         return None
     for line in open(TRAFFIC_SUMMARY):
-        if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":", 1)[1])
+        if line.startswith("traffic_bytes_per_walker_step:"):
+            return float(line.split(":", 1)[1]) * walkers_per_launch
     return None
 
 
@@ -232,7 +233,7 @@ def main():
                                f"Marsaglia(4) grid, {args.precision} contractions",
                    "walkers": N, "L": L, "D": D},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": measured_traffic(L, D, N),
+                     "frac": achieved / peak, "traffic": measured_traffic(L, D, N, walkers_per_launch),
                      "kernel": eng.kernel_name, "avg_launch_ms": avg_launch_s * 1000.0,
                      "flops_per_walker_step": F32_PER_WS + F64_PER_WS},
         "variance_x_gpu_s": None if vt is None else {

@@ -936,28 +936,47 @@ __global__ __launch_bounds__(RED_NT) void popcontrol_kernel(Book B, double host_
     if (threadIdx.x == 0) *B.n_extra = carry_extra;
 }
 
-// Copy walker data for the extra copies; then install the new order.
+// Copy walker data for the extra copies; then install the new order.  The
+// bulk (walker tensors and right environments of every copy) is spread over
+// the whole grid in 16-byte chunks where the layout allows it, so the copies
+// run at memory bandwidth instead of one workgroup per copy.
 template <typename T>
 __global__ __launch_bounds__(MQE_NT) void copy_walkers_kernel(Book B, Pool P, int L) {
-    const int e = blockIdx.x;
-    if (e >= *B.n_extra) return;
-    const int src = B.copy_src[e], dst = B.copy_dst[e];
-    const T *ps = (const T *)P.phi + (size_t)src * P.wflat;
-    T *pd = (T *)P.phi + (size_t)dst * P.wflat;
-    for (long q = threadIdx.x; q < P.wflat; q += MQE_NT) pd[q] = ps[q];
-    const T *rs = (const T *)P.Renv + (size_t)src * P.eflat;
-    T *rd = (T *)P.Renv + (size_t)dst * P.eflat;
-    for (long q = threadIdx.x; q < P.eflat; q += MQE_NT) rd[q] = rs[q];
-    for (int k = threadIdx.x; k <= L; k += MQE_NT) P.wvd[(size_t)dst * (L + 1) + k] = P.wvd[(size_t)src * (L + 1) + k];
-    if (threadIdx.x == 0) {
+    const int ne = *B.n_extra;
+    const long gid = (long)blockIdx.x * MQE_NT + threadIdx.x, gsz = (long)gridDim.x * MQE_NT;
+    for (long e = gid; e < ne; e += gsz) {  // scalars, bond dims, bond energies
+        const int src = B.copy_src[e], dst = B.copy_dst[e];
         P.w[dst] = P.w[src];
         P.O[dst] = P.O[src];
         P.eh[dst] = P.eh[src];
         P.EL[dst] = P.EL[src];
+        for (int k = 0; k <= L; ++k) P.wvd[(size_t)dst * (L + 1) + k] = P.wvd[(size_t)src * (L + 1) + k];
+        if (P.bondE)
+            for (int k = 0; k < P.nbe; ++k) P.bondE[(size_t)dst * P.nbe + k] = P.bondE[(size_t)src * P.nbe + k];
     }
-    if (P.bondE)
-        for (int k = threadIdx.x; k < P.nbe; k += MQE_NT)
-            P.bondE[(size_t)dst * P.nbe + k] = P.bondE[(size_t)src * P.nbe + k];
+    const bool vec = (P.wflat * (long)sizeof(T)) % 16 == 0 && (P.eflat * (long)sizeof(T)) % 16 == 0;
+    if (vec) {
+        const long wq = P.wflat * (long)sizeof(T) / 16, eq = P.eflat * (long)sizeof(T) / 16, per = wq + eq;
+        for (long x = gid; x < (long)ne * per; x += gsz) {
+            const long e = x / per, q = x - e * per;
+            const int src = B.copy_src[e], dst = B.copy_dst[e];
+            const float4 *s4 = q < wq ? (const float4 *)P.phi + (size_t)src * wq + q
+                                      : (const float4 *)P.Renv + (size_t)src * eq + (q - wq);
+            float4 *d4 = q < wq ? (float4 *)P.phi + (size_t)dst * wq + q : (float4 *)P.Renv + (size_t)dst * eq + (q - wq);
+            *d4 = *s4;
+        }
+    } else {
+        const long per = P.wflat + P.eflat;
+        for (long x = gid; x < (long)ne * per; x += gsz) {
+            const long e = x / per, q = x - e * per;
+            const int src = B.copy_src[e], dst = B.copy_dst[e];
+            if (q < P.wflat)
+                ((T *)P.phi)[(size_t)dst * P.wflat + q] = ((const T *)P.phi)[(size_t)src * P.wflat + q];
+            else
+                ((T *)P.Renv)[(size_t)dst * P.eflat + q - P.wflat] =
+                    ((const T *)P.Renv)[(size_t)src * P.eflat + q - P.wflat];
+        }
+    }
 }
 
 // Features of the learned proposal, row b*nC + c (MQE_NFEAT floats): log O,
@@ -1211,7 +1230,7 @@ template <typename T>
 static void launch_copy(mqe_engine *e) {
     Book B = make_book(e);
     Pool P = make_pool(e, nullptr, 0);
-    hipLaunchKernelGGL(copy_walkers_kernel<T>, dim3(e->p.capacity), dim3(MQE_NT), 0, e->stream, B, P, e->L);
+    hipLaunchKernelGGL(copy_walkers_kernel<T>, dim3(2048), dim3(MQE_NT), 0, e->stream, B, P, e->L);
 }
 
 static int enqueue_popcontrol(mqe_engine *e, double scaling, const double *du, int nu, int step) {
