@@ -41,7 +41,7 @@ def log(msg):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--steps", type=int, default=2048)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--walkers", type=int, default=2048)
     ap.add_argument("--L", type=int, default=32)
@@ -190,8 +190,8 @@ def main():
     eng.run(1, args.warmup)
     eng.sync()
     log("bench: warmup done; timing")
-    if args.steps % 32:
-        log("bench: note: the tile path re-orthonormalises every 32 steps; time a multiple of 32 steps "
+    if args.steps % 64:
+        log("bench: note: the tile path re-orthonormalises every 64 steps; time a multiple of 64 steps "
             "for the amortised rate")
     eng.set_timing(True)
     first = args.warmup + 1

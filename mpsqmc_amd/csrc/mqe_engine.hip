@@ -1089,7 +1089,7 @@ struct mqe_engine {
     int last_walkers = 0;
     hipGraphExec_t graph_exec = nullptr;
     // per-launch timing of the walker kernel inside mqe_step / mqe_run
-    int canon_every = 32;  // tile path: FP64 LQ re-canonicalisation every this many steps
+    int canon_every = 64;  // tile path: FP64 LQ re-canonicalisation every this many steps
     long prop_count = 0;
     bool timing = false;
     std::vector<hipEvent_t> tev;
