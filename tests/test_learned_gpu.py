@@ -178,3 +178,28 @@ def test_collect_train_and_run_learned():
     eng.set_proposal(None)
     eng.close()
     net.close()
+
+
+def test_learned_api_errors():
+    from mpsqmc_amd._lib import MqeError
+    s = systems.chain(8)
+    tvd, tdata = random_trial(8, 4, 41)
+    gen = Engine(s, tvd, tdata, n_desired=4, capacity=16, max_bond=4, precision="fp64")
+    assert gen.path == "general"
+    with pytest.raises(MqeError):
+        gen.collect_samples(100)
+    gen.close()
+    eng = Engine(s, tvd, tdata, n_desired=4, capacity=16, max_bond=4, precision="fp32")
+    eng.load_copies(tvd, tdata, 4)
+    eng.prepare()
+    wrong = MLP(5, 16, NLABEL, seed=1)
+    with pytest.raises(MqeError):
+        eng.set_proposal(wrong, 0.1)          # input width must be MQE_NFEAT
+    net = MLP(NFEAT, 16, NLABEL, seed=1)
+    with pytest.raises(MqeError):
+        eng.set_proposal(net, 1.5)            # mix outside [0, 1]
+    with pytest.raises(MqeError):
+        eng.train_proposal(net)               # nothing collected
+    eng.close()
+    wrong.close()
+    net.close()
