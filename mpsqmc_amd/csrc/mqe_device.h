@@ -218,64 +218,69 @@ __device__ __forceinline__ void gemm(int m, int n, int k, const T *A, int lda, c
 // ---------------------------------------------------------------------------
 // Householder QR (LAPACK geqrf/orgqr conventions), FP64, whole workgroup.
 // A: m x n (m >= n), ld m.  On exit A holds Q and R (n x n, ld n) the upper
-// triangle.  tau: n; parts: MQE_NT; dots: n; sh: MQE_WAVES.
+// triangle.  tau: n.  (parts, dots, sh: unused, kept for the call sites.)
+// Column c belongs to wave c % 4 for the whole factorisation: the owner of
+// column j forms reflector j from its (already updated) column, one
+// workgroup barrier publishes it, and every wave applies it to its own
+// columns, four columns at a time, with the 64 lanes running down the rows
+// (coalesced) and wave reductions for the dot products.  Q is accumulated
+// backwards the same way (one barrier per reflector).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void apply_reflector(int m, int n, int j, double tj, double *A, double *parts,
-                                                double *dots) {
-    const int tid = threadIdx.x;
-    const int c0 = j + 1, nc = n - c0;
-    if (nc <= 0) return;
-    const int G = max(1, MQE_NT / nc);
-    if (tid < G * nc) {
-        const int cc = tid % nc, g = tid / nc;
-        const double *vj = A + (size_t)m * j;
-        const double *ac = A + (size_t)m * (c0 + cc);
-        double s = 0.0;
-        for (int i = j + 1 + g; i < m; i += G) s += vj[i] * ac[i];
-        parts[g * nc + cc] = s;
+__device__ __forceinline__ void reflect_own_columns(int m, int n, int j, double tj, double *A, int w, int lane) {
+    const double *vj = A + (size_t)m * j;
+    int c = j + 1 + ((w - (j + 1)) & (MQE_WAVES - 1));  // first column > j owned by this wave
+    for (; c < n; c += 4 * MQE_WAVES) {
+        double s[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int i = j + lane; i < m; i += 64) {
+            const double v = i == j ? 1.0 : vj[i];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (c + MQE_WAVES * q < n) s[q] += v * A[i + (size_t)m * (c + MQE_WAVES * q)];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) s[q] = tj * wave_sum(s[q]);
+        for (int i = j + lane; i < m; i += 64) {
+            const double v = i == j ? 1.0 : vj[i];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (c + MQE_WAVES * q < n) A[i + (size_t)m * (c + MQE_WAVES * q)] -= s[q] * v;
+        }
     }
-    __syncthreads();
-    if (tid < nc) {
-        double s = A[j + (size_t)m * (c0 + tid)];
-        for (int g = 0; g < G; ++g) s += parts[g * nc + tid];
-        dots[tid] = tj * s;
-    }
-    __syncthreads();
-    const int rows = m - j;
-    for (int e = tid; e < nc * rows; e += MQE_NT) {
-        const int cc = e % nc, i = j + e / nc;
-        const double f = dots[cc];
-        A[i + (size_t)m * (c0 + cc)] -= f * (i == j ? 1.0 : A[i + (size_t)m * j]);
-    }
-    __syncthreads();
 }
 
 __device__ __forceinline__ void householder_qr(int m, int n, double *A, double *R, double *tau, double *parts,
                                                double *dots, double *sh) {
-    const int tid = threadIdx.x;
+    (void)parts;
+    (void)dots;
+    (void)sh;
+    const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
     for (int j = 0; j < n; ++j) {
-        double ss = 0.0;
-        for (int i = j + 1 + tid; i < m; i += MQE_NT) {
-            const double a = A[i + (size_t)m * j];
-            ss += a * a;
-        }
-        ss = block_sum(ss, sh);
-        const double alpha = A[j + (size_t)m * j];
-        double tj = 0.0, beta = alpha, scale = 0.0;
-        if (ss != 0.0) {
-            beta = -copysign(sqrt(alpha * alpha + ss), alpha);
-            tj = (beta - alpha) / beta;
-            scale = 1.0 / (alpha - beta);
+        if (w == (j & (MQE_WAVES - 1))) {
+            double *aj = A + (size_t)m * j;
+            double ss = 0.0;
+            for (int i = j + 1 + lane; i < m; i += 64) {
+                const double a = aj[i];
+                ss += a * a;
+            }
+            ss = wave_sum(ss);
+            const double alpha = aj[j];
+            double tj = 0.0, beta = alpha, scale = 0.0;
+            if (ss != 0.0) {
+                beta = -copysign(sqrt(alpha * alpha + ss), alpha);
+                tj = (beta - alpha) / beta;
+                scale = 1.0 / (alpha - beta);
+            }
+            if (tj != 0.0) {
+                for (int i = j + 1 + lane; i < m; i += 64) aj[i] *= scale;
+                if (lane == 0) aj[j] = beta;
+            }
+            if (lane == 0) tau[j] = tj;
         }
         __syncthreads();
-        if (tj != 0.0) {
-            for (int i = j + 1 + tid; i < m; i += MQE_NT) A[i + (size_t)m * j] *= scale;
-            if (tid == 0) A[j + (size_t)m * j] = beta;
-        }
-        if (tid == 0) tau[j] = tj;
-        __syncthreads();
-        if (tj != 0.0) apply_reflector(m, n, j, tj, A, parts, dots);
+        const double tj = tau[j];
+        if (tj != 0.0) reflect_own_columns(m, n, j, tj, A, w, lane);
     }
+    __syncthreads();
     for (int e = tid; e < n * n; e += MQE_NT) {
         const int r = e % n, c = e / n;
         R[e] = (r <= c) ? A[r + (size_t)m * c] : 0.0;
@@ -283,13 +288,14 @@ __device__ __forceinline__ void householder_qr(int m, int n, double *A, double *
     __syncthreads();
     for (int j = n - 1; j >= 0; --j) {
         const double tj = tau[j];
-        if (tj != 0.0) apply_reflector(m, n, j, tj, A, parts, dots);
-        for (int i = tid; i < m; i += MQE_NT) {
-            const double v = A[i + (size_t)m * j];
-            A[i + (size_t)m * j] = (i < j) ? 0.0 : (i == j ? 1.0 - tj : -tj * v);
+        if (tj != 0.0) reflect_own_columns(m, n, j, tj, A, w, lane);
+        __syncthreads();  // every wave has read reflector j
+        if (w == (j & (MQE_WAVES - 1))) {
+            double *aj = A + (size_t)m * j;
+            for (int i = lane; i < m; i += 64) aj[i] = (i < j) ? 0.0 : (i == j ? 1.0 - tj : -tj * aj[i]);
         }
-        __syncthreads();
     }
+    __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
