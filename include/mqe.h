@@ -100,8 +100,11 @@ int mqe_population_control(mqe_engine *e, double scaling, const double *uniforms
  * Omega, sum_w) is kept on the device and read with mqe_read_trace. */
 int mqe_step(mqe_engine *e, int step);
 
-/* Run steps first_step .. first_step+n_steps-1 with mqe_step; when
- * use_graph != 0 the step is captured once in a hipGraph and replayed. */
+/* Run steps first_step .. first_step+n_steps-1 with mqe_step.  use_graph is
+ * accepted and ignored: the launches of a step carry its number (the device
+ * RNG key) and the re-orthonormalisation flag as arguments, and the host
+ * enqueues them asynchronously far ahead of the GPU (~8 launches per
+ * multi-millisecond step), so a replayed graph would save no GPU time. */
 int mqe_run(mqe_engine *e, int first_step, int n_steps, int use_graph);
 
 /* Copy the trace rows of steps first_step .. first_step+n_steps-1 (6 doubles each). */
