@@ -84,6 +84,7 @@ struct Pool {
     // learned proposal (tile path)
     double *bondE;         // per slot: nC bond energies <trial|h_c|phi> and <trial|phi> (or nullptr)
     int nbe;               // entries of bondE per slot (nC + 1)
+    int qlds;              // general path: the LQ workspace is in dynamic LDS
     const float *pred;     // predicted labels, row b*nC + c, 16 each (or nullptr)
     double mix;            // weight of the uniform part of the learned proposal
     float *lab;            // label rows of the sample buffer (or nullptr)
@@ -146,6 +147,7 @@ struct Ctx {
     T *ts;               // T scratch
     double *ds;          // double scratch
     double *red;         // LDS reduction scratch (>= MQE_NT + 16*MQE_WAVES)
+    double *qlds;        // LDS workspace of the LQ (2 Dw Dw + Dw doubles), or nullptr: global scratch
 
     __device__ T *site(int k) const { return phi + woff[k]; }
     __device__ const T *tsite(int k) const { return trial + S->toff[k]; }
@@ -342,14 +344,17 @@ template <typename T>
 __device__ void lq_site(const Ctx<T> &C, int k) {
     const int wk = C.wv[k], wn = C.wv[k + 1], wp = C.wv[k - 1];
     const int m = 2 * wn, n = wk;
-    double *A = C.ds + C.sl.AQ, *R = C.ds + C.sl.RQ;
+    // the FP64 factorisation works in LDS when it fits (its sequential column
+    // steps are latency-bound), otherwise in the global scratch
+    double *A = C.qlds ? C.qlds : C.ds + C.sl.AQ, *R = C.ds + C.sl.RQ;
+    double *tau = C.qlds ? C.qlds + (size_t)2 * C.S->Dw * C.S->Dw : C.ds + C.sl.TAU;
     T *ph = C.site(k);
     for (int e = threadIdx.x; e < m * n; e += MQE_NT) {
         const int r = e % m, c = e / m;
         A[e] = (double)ph[c + (size_t)wk * r];
     }
     __syncthreads();
-    householder_qr(m, n, A, R, C.ds + C.sl.TAU, C.ds + C.sl.PARTS, C.ds + C.sl.DOTS, C.red);
+    householder_qr(m, n, A, R, tau, C.ds + C.sl.PARTS, C.ds + C.sl.DOTS, C.red);
     for (int e = threadIdx.x; e < m * n; e += MQE_NT) {
         const int r = e % m, c = e / m;
         ph[c + (size_t)wk * r] = (T)A[e];
@@ -545,6 +550,8 @@ __global__ __launch_bounds__(MQE_NT) void walker_kernel(Sys S, Pool P, const T *
     long tb = (C.sl.t_total * (long)sizeof(T) + 255) & ~255L;
     C.ds = (double *)(base + tb);
     C.red = red;
+    extern __shared__ double qlds_dyn[];
+    C.qlds = P.qlds ? qlds_dyn : nullptr;
 
     double w = P.w[slot];
     double O = P.O[slot];
@@ -1069,6 +1076,7 @@ struct mqe_engine {
     char *d_scratch = nullptr;
     int *d_status = nullptr;
     // learned proposal (tile path)
+    size_t gen_lds_bytes = 0;                  // general path: dynamic LDS of the LQ workspace (0: global scratch)
     double *d_bond = nullptr;                  // per slot: nC bond energies and the overlap of the last sweep
     float *d_feat = nullptr, *d_pred = nullptr;  // rows b*nC + c of the current step
     float *d_sX = nullptr, *d_sY = nullptr;    // collected samples (features, labels)
@@ -1121,6 +1129,7 @@ static Pool make_pool(mqe_engine *e, const double *u, int step) {
     P.scratch = e->d_scratch;
     P.scratch_bytes = e->scratch_bytes;
     P.padded = e->tile32 ? 1 : 0;
+    P.qlds = e->gen_lds_bytes > 0 ? 1 : 0;
     P.bondE = e->d_bond;
     P.nbe = e->nC + 1;
     P.pred = e->learn ? e->d_pred : nullptr;
@@ -1161,7 +1170,7 @@ static Book make_book(mqe_engine *e) {
 template <typename T>
 static void launch_walker(mqe_engine *e, const double *u, int step, int flags) {
     Pool P = make_pool(e, u, step);
-    hipLaunchKernelGGL(walker_kernel<T>, dim3(e->p.capacity), dim3(MQE_NT), 0, e->stream, e->sys, P,
+    hipLaunchKernelGGL(walker_kernel<T>, dim3(e->p.capacity), dim3(MQE_NT), e->gen_lds_bytes, e->stream, e->sys, P,
                        (const T *)e->d_trial, flags);
 }
 
@@ -1358,6 +1367,18 @@ int mqe_create(const mqe_params *p, const mqe_mps *trial, int n_couplings, const
         CK(hipMemset(e->d_bond, 0, sizeof(double) * (size_t)p->capacity * (n_couplings + 1)));
         CK(hipMalloc((void **)&e->d_nsamp, sizeof(long)));
         CK(hipMemset(e->d_nsamp, 0, sizeof(long)));
+    }
+    if (!e->tile32) {
+        // the general kernel runs one workgroup per CU (its register use); its
+        // LQ workspace (2 Dw x Dw matrix + Dw tau) goes to LDS when it fits
+        const size_t need = sizeof(double) * ((size_t)2 * e->Dw * e->Dw + e->Dw);
+        if (need <= 144 * 1024) {
+            e->gen_lds_bytes = need;
+            CK(hipFuncSetAttribute((const void *)walker_kernel<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)need));
+            CK(hipFuncSetAttribute((const void *)walker_kernel<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)need));
+        }
     }
     const int nC = n_couplings;
     std::vector<int> nextfirst(std::max(nC, 1)), csptr(L + 2, 0), csidx(std::max(nC, 1));
