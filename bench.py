@@ -54,7 +54,7 @@ def parse():
     ap.add_argument("--random-trial", action="store_true", help="random trial instead of the DMRG one")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal comparison")
     ap.add_argument("--learned-samples", type=int, default=1_000_000)
-    ap.add_argument("--learned-epochs", type=int, default=4)
+    ap.add_argument("--learned-epochs", type=int, default=16)
     ap.add_argument("--learned-mix", type=float, default=0.01,
                     help="weight of the uniform part of the learned proposal (keeps its support)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",

@@ -166,7 +166,9 @@ const char *mqe_last_error(void);
  *
  * Features of walker b at coupling c (MQE_NFEAT floats, from the start of the
  * step): log O, log w, E_L, e_{c-1}, e_c, e_{c+1}, c/(nC-1), with e_k the
- * local energy of bond k from the previous energy sweep (0 beyond the ends).
+ * local energy of bond k from the previous energy sweep (0 beyond the ends),
+ * then the 16 labels of the same coupling in the walker's previous
+ * propagation with collection or the learned proposal on (0 before that).
  * Labels: the 16 normalised two-site elements combos[a+4b]/O of the coupling
  * (MPSQMC2.cpp:540-549), so that ratio(x) = sum_ab label[a+4b] x_a x_b.
  *
@@ -178,13 +180,13 @@ const char *mqe_last_error(void);
  * (unbiased).  If pdf(x) = 0 the weight becomes 0 and the walker continues
  * with the reference's own sample.  Tile path only (nearest-neighbour chains,
  * bonds <= 32, FP32). */
-#define MQE_NFEAT 7
+#define MQE_NFEAT 23
 #define MQE_NLABEL 16
 
 typedef struct mqe_mlp mqe_mlp;
 
 /* MLP n_in -> hidden -> hidden -> n_out with ReLU, FP32, He-normal weights
- * drawn from std::mt19937_64(seed).  n_in <= 16, hidden <= 128, n_out <= 32. */
+ * drawn from std::mt19937_64(seed).  n_in <= 32, hidden <= 128, n_out <= 32. */
 int mqe_mlp_create(int n_in, int hidden, int n_out, unsigned long long seed, mqe_mlp **out);
 void mqe_mlp_destroy(mqe_mlp *m);
 long mqe_mlp_num_params(const mqe_mlp *m);

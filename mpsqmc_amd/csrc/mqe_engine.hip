@@ -88,6 +88,8 @@ struct Pool {
     const float *pred;     // predicted labels, row b*nC + c, 16 each (or nullptr)
     double mix;            // weight of the uniform part of the learned proposal
     float *lab;            // label rows of the sample buffer (or nullptr)
+    float *plab;           // per slot and coupling: the labels of the last recorded propagation (16 each)
+    int plab_on;           // the walker kernel records them in this launch
     const long *nsamp;     // rows already in the sample buffer
     long maxsamp;          // its capacity in rows
 };
@@ -961,6 +963,13 @@ __global__ __launch_bounds__(MQE_NT) void copy_walkers_kernel(Book B, Pool P, in
         if (P.bondE)
             for (int k = 0; k < P.nbe; ++k) P.bondE[(size_t)dst * P.nbe + k] = P.bondE[(size_t)src * P.nbe + k];
     }
+    if (P.plab) {
+        const long per = 16L * (P.nbe - 1);
+        for (long x = gid; x < (long)ne * per; x += gsz) {
+            const long e = x / per, q = x - e * per;
+            P.plab[(size_t)B.copy_dst[e] * per + q] = P.plab[(size_t)B.copy_src[e] * per + q];
+        }
+    }
     const bool vec = (P.wflat * (long)sizeof(T)) % 16 == 0 && (P.eflat * (long)sizeof(T)) % 16 == 0;
     if (vec) {
         const long wq = P.wflat * (long)sizeof(T) / 16, eq = P.eflat * (long)sizeof(T) / 16, per = wq + eq;
@@ -988,7 +997,7 @@ __global__ __launch_bounds__(MQE_NT) void copy_walkers_kernel(Book B, Pool P, in
 
 // Features of the learned proposal, row b*nC + c (MQE_NFEAT floats): log O,
 // log w, E_L, e_{c-1}, e_c, e_{c+1}, c/(nC-1) with e_k = bond energy / overlap
-// of the last energy sweep.  X: rows of this step (or nullptr); the sample
+// of the last energy sweep, then the walker's 16 recorded labels of coupling c.  X: rows of this step (or nullptr); the sample
 // buffer gets the same rows after its *nsamp rows if they fit.
 __global__ void feature_kernel(Pool P, int nC, float *X, float *ring) {
     const int n = *P.n;
@@ -1002,6 +1011,8 @@ __global__ void feature_kernel(Pool P, int nC, float *X, float *ring) {
         for (int c = threadIdx.x; c < nC; c += blockDim.x) {
             float f[MQE_NFEAT] = {lO, lw, EL, c > 0 ? (float)(be[c - 1] * inv) : 0.f, (float)(be[c] * inv),
                                   c + 1 < nC ? (float)(be[c + 1] * inv) : 0.f, nC > 1 ? (float)c / (nC - 1) : 0.f};
+            const float *pl = P.plab + ((size_t)slot * nC + c) * 16;
+            for (int q = 0; q < 16; ++q) f[7 + q] = pl[q];
             const long row = (long)b * nC + c;
             for (int q = 0; q < MQE_NFEAT; ++q) {
                 if (X) X[row * MQE_NFEAT + q] = f[q];
@@ -1080,6 +1091,7 @@ struct mqe_engine {
     double *d_bond = nullptr;                  // per slot: nC bond energies and the overlap of the last sweep
     float *d_feat = nullptr, *d_pred = nullptr;  // rows b*nC + c of the current step
     float *d_sX = nullptr, *d_sY = nullptr;    // collected samples (features, labels)
+    float *d_plab = nullptr;                   // per slot: labels of the last recorded propagation
     long *d_nsamp = nullptr;
     long max_samples = 0;
     mqe_mlp *net = nullptr;
@@ -1135,6 +1147,8 @@ static Pool make_pool(mqe_engine *e, const double *u, int step) {
     P.pred = e->learn ? e->d_pred : nullptr;
     P.mix = e->mix;
     P.lab = e->max_samples > 0 ? e->d_sY : nullptr;
+    P.plab = e->d_plab;
+    P.plab_on = (e->max_samples > 0 || e->learn) ? 1 : 0;
     P.nsamp = e->d_nsamp;
     P.maxsamp = e->max_samples;
     return P;
@@ -1367,6 +1381,8 @@ int mqe_create(const mqe_params *p, const mqe_mps *trial, int n_couplings, const
         CK(hipMemset(e->d_bond, 0, sizeof(double) * (size_t)p->capacity * (n_couplings + 1)));
         CK(hipMalloc((void **)&e->d_nsamp, sizeof(long)));
         CK(hipMemset(e->d_nsamp, 0, sizeof(long)));
+        CK(hipMalloc((void **)&e->d_plab, sizeof(float) * (size_t)p->capacity * std::max(n_couplings, 1) * 16));
+        CK(hipMemset(e->d_plab, 0, sizeof(float) * (size_t)p->capacity * std::max(n_couplings, 1) * 16));
     }
     if (!e->tile32) {
         // the general kernel runs one workgroup per CU (its register use); its
@@ -1499,7 +1515,7 @@ void mqe_destroy(mqe_engine *e) {
                     e->d_Renv, e->d_Lenv, e->d_wvd, e->d_w, e->d_O, e->d_eh, e->d_EL, e->d_order, e->d_order2,
                     e->d_n, e->d_idx, e->d_u, e->d_scratch, e->d_status, e->d_sums, e->d_trace, e->d_ncopy,
                     e->d_newpos, e->d_used, e->d_free, e->d_csrc, e->d_cdst, e->d_nextra, e->d_nnew, e->d_nused,
-                    e->d_tmpout, e->d_bond, e->d_feat, e->d_pred, e->d_sX, e->d_sY, e->d_nsamp};
+                    e->d_tmpout, e->d_bond, e->d_feat, e->d_pred, e->d_sX, e->d_sY, e->d_nsamp, e->d_plab};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     for (hipEvent_t ev : e->tev) (void)hipEventDestroy(ev);

@@ -34,7 +34,7 @@ namespace mqe {
 namespace mlp {
 
 constexpr int TS = 64;         // samples per tile
-constexpr int MAX_IN = 16, MAX_H = 128, MAX_OUT = 32;
+constexpr int MAX_IN = 32, MAX_H = 128, MAX_OUT = 32;
 
 struct Net {
     int nin, H, nout;
@@ -350,7 +350,7 @@ extern "C" {
 int mqe_mlp_create(int n_in, int hidden, int n_out, unsigned long long seed, mqe_mlp **out) {
     if (!out) { set_error("mqe_mlp_create: null argument"); return MQE_EINVAL; }
     if (n_in < 1 || n_in > mlp::MAX_IN || hidden < 1 || hidden > mlp::MAX_H || n_out < 1 || n_out > mlp::MAX_OUT) {
-        set_error("mqe_mlp_create: sizes out of range (n_in <= 16, hidden <= 128, n_out <= 32)");
+        set_error("mqe_mlp_create: sizes out of range (n_in <= 32, hidden <= 128, n_out <= 32)");
         return MQE_EINVAL;
     }
     mqe_mlp *m = new mqe_mlp();

@@ -816,9 +816,13 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
 #pragma unroll
         for (int q = 0; q < 16; ++q) cb[q] = readlane_d(cq, q);
         const double pref = 4.0 / (O * Ng);
-        if (P.lab && w == 0 && lane < 16) {  // training label combos/O
-            const long base = *P.nsamp, row = base + (long)b * S.nC + c;
-            if (base + (long)n * S.nC <= P.maxsamp) P.lab[row * 16 + lane] = (float)(cq / O);
+        if (P.plab_on && w == 0 && lane < 16) {  // labels combos/O: the walker's record and the training rows
+            const float lv = (float)(cq / O);
+            P.plab[((size_t)slot * S.nC + c) * 16 + lane] = lv;
+            if (P.lab) {
+                const long base = *P.nsamp, row = base + (long)b * S.nC + c;
+                if (base + (long)n * S.nC <= P.maxsamp) P.lab[row * 16 + lane] = lv;
+            }
         }
         double pv[4], loc = 0.0;
 #pragma unroll

@@ -2,7 +2,7 @@
 
 An MLP n_in -> hidden -> hidden -> n_out with ReLU in FP32 whose forward,
 backward and Adam kernels run on the GPU (``csrc/mqe_mlp.hip``).  For the
-learned proposal it maps MQE_NFEAT = 7 per-walker, per-coupling features to the
+learned proposal it maps MQE_NFEAT = 23 per-walker, per-coupling features to the
 MQE_NLABEL = 16 normalised two-site elements combos/O; see
 ``Engine.collect_samples``, ``Engine.train_proposal`` and
 ``Engine.set_proposal``.  The reference has no learned component.
@@ -13,7 +13,7 @@ import numpy as np
 
 from ._lib import check, lib
 
-NFEAT = 7
+NFEAT = 23
 NLABEL = 16
 
 _fp = ctypes.POINTER(ctypes.c_float)

@@ -52,15 +52,16 @@ def test_mlp_gradient_matches_fp64_restatement():
 
 def test_mlp_fit_learns_a_smooth_map():
     rng = np.random.default_rng(7)
-    A = rng.normal(size=(NFEAT, 24))
+    nin = 7
+    A = rng.normal(size=(nin, 24))
     B = rng.normal(size=(24, NLABEL)) / 5
-    X = rng.normal(size=(20000, NFEAT)).astype(np.float32)
+    X = rng.normal(size=(20000, nin)).astype(np.float32)
     Y = (np.tanh(X @ A) @ B).astype(np.float32)
-    net = MLP(NFEAT, H, NLABEL, seed=8)
+    net = MLP(nin, H, NLABEL, seed=8)
     loss = net.fit(X, Y, epochs=30, batch=512, lr=2e-3, standardize=True)
     # standardised targets have unit variance: the fit explains > 95 % of it
     assert loss < 0.05
-    Xt = rng.normal(size=(500, NFEAT)).astype(np.float32)
+    Xt = rng.normal(size=(500, nin)).astype(np.float32)
     Yt = np.tanh(Xt @ A) @ B
     assert np.mean((net.forward(Xt) - Yt) ** 2) < 0.1 * np.var(Yt)
     net.close()
@@ -119,6 +120,11 @@ def test_engine_training_rows_match_oracle():
     assert np.allclose(F[:, 1:, 3], be[:, :-1], rtol=1e-5, atol=1e-6) and np.all(F[:, 0, 3] == 0)
     assert np.allclose(F[:, :-1, 5], be[:, 1:], rtol=1e-5, atol=1e-6) and np.all(F[:, -1, 5] == 0)
     assert np.allclose(F[0, :, 6], np.arange(nC) / (nC - 1))
+    assert np.all(F[:, :, 7:] == 0)  # no recorded labels before the first collected step
+    # a second step: its features carry the labels each walker recorded in the first
+    eng.propagate(np.random.default_rng(23).uniform(size=16 * nC))
+    X2, _ = eng.read_samples(16 * nC, 16 * nC)
+    assert np.array_equal(X2.reshape(16, nC, NFEAT)[:, :, 7:], Y.reshape(16, nC, 16))
     eng.close()
 
 
