@@ -161,7 +161,8 @@ def learned_comparison(eng, args, first, plain_vt):
             "population_mean": float(tr[:, 1].mean()),
             "variance_x_gpu_s": vt["variance_x_seconds"], "E_mean": vt["mean"], "E_error": vt["error"],
             "block_length": vt["block_length"], "steps": args.steps, "seconds": wall,
-            "gain_vs_plain": (plain_vt["variance_x_seconds"] / vt["variance_x_seconds"]) if plain_vt else None}
+            "gain_vs_plain": (plain_vt["variance_x_seconds"] / vt["variance_x_seconds"]) if plain_vt else None,
+            "gain_rel_error": float(np.hypot(plain_vt["var_rel_error"], vt["var_rel_error"])) if plain_vt else None}
 
 
 def main():

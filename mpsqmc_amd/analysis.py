@@ -44,8 +44,9 @@ def blocking(x, min_blocks=16):
         if all(v2 - v <= dv + dv2 for (_, _, v2, dv2) in levels[i + 1:]):
             pick = levels[i]
             break
-    length, n, v, _ = pick
+    length, n, v, dv = pick
     return {"mean": float(x.mean()), "error": float(np.sqrt(v)), "var_mean": float(v),
+            "var_rel_error": float(dv / v) if v > 0 else float("inf"),
             "block_length": int(length), "n": int(x.size)}
 
 
