@@ -217,8 +217,10 @@ __device__ __forceinline__ void gemm(int m, int n, int k, const T *A, int lda, c
 
 // ---------------------------------------------------------------------------
 // Householder QR (LAPACK geqrf/orgqr conventions), FP64, whole workgroup.
-// A: m x n (m >= n), ld m.  On exit A holds Q and R (n x n, ld n) the upper
-// triangle.  tau: n.  (parts, dots, sh: unused, kept for the call sites.)
+// A: m x n (m >= n), column c at Ac(c) (columns c < c_split at A + m c, the
+// others at B + m (c - c_split): the caller may keep the trailing columns,
+// which receive most of the updates, in LDS).  On exit the columns hold Q and
+// R (n x n, ld n) the upper triangle.  tau: n.
 // Column c belongs to wave c % 4 for the whole factorisation: the owner of
 // column j forms reflector j from its (already updated) column, one
 // workgroup barrier publishes it, and every wave applies it to its own
@@ -226,16 +228,26 @@ __device__ __forceinline__ void gemm(int m, int n, int k, const T *A, int lda, c
 // (coalesced) and wave reductions for the dot products.  Q is accumulated
 // backwards the same way (one barrier per reflector).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void reflect_own_columns(int m, int n, int j, double tj, double *A, int w, int lane) {
-    const double *vj = A + (size_t)m * j;
+struct QRCols {
+    double *A, *B;
+    int m, c_split;
+    __device__ double *operator()(int c) const { return c < c_split ? A + (size_t)m * c : B + (size_t)m * (c - c_split); }
+};
+
+__device__ __forceinline__ void reflect_own_columns(const QRCols &col, int n, int j, double tj, int w, int lane) {
+    const int m = col.m;
+    const double *vj = col(j);
     int c = j + 1 + ((w - (j + 1)) & (MQE_WAVES - 1));  // first column > j owned by this wave
     for (; c < n; c += 4 * MQE_WAVES) {
+        double *cp[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) cp[q] = col(c + MQE_WAVES * q < n ? c + MQE_WAVES * q : c);
         double s[4] = {0.0, 0.0, 0.0, 0.0};
         for (int i = j + lane; i < m; i += 64) {
             const double v = i == j ? 1.0 : vj[i];
 #pragma unroll
             for (int q = 0; q < 4; ++q)
-                if (c + MQE_WAVES * q < n) s[q] += v * A[i + (size_t)m * (c + MQE_WAVES * q)];
+                if (c + MQE_WAVES * q < n) s[q] += v * cp[q][i];
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) s[q] = tj * wave_sum(s[q]);
@@ -243,20 +255,17 @@ __device__ __forceinline__ void reflect_own_columns(int m, int n, int j, double 
             const double v = i == j ? 1.0 : vj[i];
 #pragma unroll
             for (int q = 0; q < 4; ++q)
-                if (c + MQE_WAVES * q < n) A[i + (size_t)m * (c + MQE_WAVES * q)] -= s[q] * v;
+                if (c + MQE_WAVES * q < n) cp[q][i] -= s[q] * v;
         }
     }
 }
 
-__device__ __forceinline__ void householder_qr(int m, int n, double *A, double *R, double *tau, double *parts,
-                                               double *dots, double *sh) {
-    (void)parts;
-    (void)dots;
-    (void)sh;
+__device__ __forceinline__ void householder_qr(const QRCols &col, int n, double *R, double *tau) {
+    const int m = col.m;
     const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
     for (int j = 0; j < n; ++j) {
         if (w == (j & (MQE_WAVES - 1))) {
-            double *aj = A + (size_t)m * j;
+            double *aj = col(j);
             double ss = 0.0;
             for (int i = j + 1 + lane; i < m; i += 64) {
                 const double a = aj[i];
@@ -278,20 +287,20 @@ __device__ __forceinline__ void householder_qr(int m, int n, double *A, double *
         }
         __syncthreads();
         const double tj = tau[j];
-        if (tj != 0.0) reflect_own_columns(m, n, j, tj, A, w, lane);
+        if (tj != 0.0) reflect_own_columns(col, n, j, tj, w, lane);
     }
     __syncthreads();
     for (int e = tid; e < n * n; e += MQE_NT) {
         const int r = e % n, c = e / n;
-        R[e] = (r <= c) ? A[r + (size_t)m * c] : 0.0;
+        R[e] = (r <= c) ? col(c)[r] : 0.0;
     }
     __syncthreads();
     for (int j = n - 1; j >= 0; --j) {
         const double tj = tau[j];
-        if (tj != 0.0) reflect_own_columns(m, n, j, tj, A, w, lane);
+        if (tj != 0.0) reflect_own_columns(col, n, j, tj, w, lane);
         __syncthreads();  // every wave has read reflector j
         if (w == (j & (MQE_WAVES - 1))) {
-            double *aj = A + (size_t)m * j;
+            double *aj = col(j);
             for (int i = lane; i < m; i += 64) aj[i] = (i < j) ? 0.0 : (i == j ? 1.0 - tj : -tj * aj[i]);
         }
     }

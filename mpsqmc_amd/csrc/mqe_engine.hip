@@ -84,7 +84,7 @@ struct Pool {
     // learned proposal (tile path)
     double *bondE;         // per slot: nC bond energies <trial|h_c|phi> and <trial|phi> (or nullptr)
     int nbe;               // entries of bondE per slot (nC + 1)
-    int qlds;              // general path: the LQ workspace is in dynamic LDS
+    int qlds;              // general path: doubles of dynamic LDS for the LQ workspace (0: none)
     const float *pred;     // predicted labels, row b*nC + c, 16 each (or nullptr)
     double mix;            // weight of the uniform part of the learned proposal
     float *lab;            // label rows of the sample buffer (or nullptr)
@@ -149,7 +149,8 @@ struct Ctx {
     T *ts;               // T scratch
     double *ds;          // double scratch
     double *red;         // LDS reduction scratch (>= MQE_NT + 16*MQE_WAVES)
-    double *qlds;        // LDS workspace of the LQ (2 Dw Dw + Dw doubles), or nullptr: global scratch
+    double *qlds;        // LDS workspace of the LQ (tau, then trailing columns), or nullptr
+    long qlds_doubles;   // its size
 
     __device__ T *site(int k) const { return phi + woff[k]; }
     __device__ const T *tsite(int k) const { return trial + S->toff[k]; }
@@ -346,20 +347,24 @@ template <typename T>
 __device__ void lq_site(const Ctx<T> &C, int k) {
     const int wk = C.wv[k], wn = C.wv[k + 1], wp = C.wv[k - 1];
     const int m = 2 * wn, n = wk;
-    // the FP64 factorisation works in LDS when it fits (its sequential column
-    // steps are latency-bound), otherwise in the global scratch
-    double *A = C.qlds ? C.qlds : C.ds + C.sl.AQ, *R = C.ds + C.sl.RQ;
-    double *tau = C.qlds ? C.qlds + (size_t)2 * C.S->Dw * C.S->Dw : C.ds + C.sl.TAU;
+    // the FP64 factorisation's sequential column steps are latency-bound: the
+    // trailing columns (which receive most of the updates) live in LDS, as
+    // many as fit, the rest in the global scratch
+    double *R = C.ds + C.sl.RQ;
+    double *tau = C.qlds ? C.qlds : C.ds + C.sl.TAU;
+    int nl = 0;
+    if (C.qlds) nl = min(n, (int)((C.qlds_doubles - C.S->Dw) / m));
+    const QRCols col{C.ds + C.sl.AQ, C.qlds ? C.qlds + C.S->Dw : nullptr, m, n - nl};
     T *ph = C.site(k);
     for (int e = threadIdx.x; e < m * n; e += MQE_NT) {
         const int r = e % m, c = e / m;
-        A[e] = (double)ph[c + (size_t)wk * r];
+        col(c)[r] = (double)ph[c + (size_t)wk * r];
     }
     __syncthreads();
-    householder_qr(m, n, A, R, tau, C.ds + C.sl.PARTS, C.ds + C.sl.DOTS, C.red);
+    householder_qr(col, n, R, tau);
     for (int e = threadIdx.x; e < m * n; e += MQE_NT) {
         const int r = e % m, c = e / m;
-        ph[c + (size_t)wk * r] = (T)A[e];
+        ph[c + (size_t)wk * r] = (T)col(c)[r];
     }
     T *LM = C.ts + C.sl.LM;
     for (int e = threadIdx.x; e < n * n; e += MQE_NT) {
@@ -553,7 +558,8 @@ __global__ __launch_bounds__(MQE_NT) void walker_kernel(Sys S, Pool P, const T *
     C.ds = (double *)(base + tb);
     C.red = red;
     extern __shared__ double qlds_dyn[];
-    C.qlds = P.qlds ? qlds_dyn : nullptr;
+    C.qlds = P.qlds > 0 ? qlds_dyn : nullptr;
+    C.qlds_doubles = P.qlds;
 
     double w = P.w[slot];
     double O = P.O[slot];
@@ -1141,7 +1147,7 @@ static Pool make_pool(mqe_engine *e, const double *u, int step) {
     P.scratch = e->d_scratch;
     P.scratch_bytes = e->scratch_bytes;
     P.padded = e->tile32 ? 1 : 0;
-    P.qlds = e->gen_lds_bytes > 0 ? 1 : 0;
+    P.qlds = (int)(e->gen_lds_bytes / sizeof(double));
     P.bondE = e->d_bond;
     P.nbe = e->nC + 1;
     P.pred = e->learn ? e->d_pred : nullptr;
@@ -1386,9 +1392,10 @@ int mqe_create(const mqe_params *p, const mqe_mps *trial, int n_couplings, const
     }
     if (!e->tile32) {
         // the general kernel runs one workgroup per CU (its register use); its
-        // LQ workspace (2 Dw x Dw matrix + Dw tau) goes to LDS when it fits
-        const size_t need = sizeof(double) * ((size_t)2 * e->Dw * e->Dw + e->Dw);
-        if (need <= 144 * 1024) {
+        // LQ workspace (Dw tau, then up to 2 Dw x Dw matrix columns) goes to LDS,
+        // as much as 144 KB holds
+        const size_t need = std::min(sizeof(double) * ((size_t)2 * e->Dw * e->Dw + e->Dw), (size_t)144 * 1024);
+        if (need >= sizeof(double) * (size_t)(e->Dw + 2 * e->Dw)) {
             e->gen_lds_bytes = need;
             CK(hipFuncSetAttribute((const void *)walker_kernel<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)need));
