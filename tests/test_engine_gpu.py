@@ -284,3 +284,30 @@ def test_tile32_parity_chain32_d32_fp32():
     EP_o, _, EL_o = orc.energy(1, True)
     assert rel(eng.scalars()["EL"], EL_o) < 1e-3
     eng.close()
+
+
+def test_parity_large_bond_fp64():
+    # bond 128 in the middle of an L = 14 chain: the 256 x 128 FP64 matrix of the LQ
+    # exceeds the LDS workspace, so its leading columns stay in global scratch
+    import os
+    s = systems.chain(14)
+    tvd, tdata = random_trial(14, 128, 5)
+    orc = oracle_population(s, tvd, tdata, threads=min(16, os.cpu_count() or 1))
+    vds, datas = zip(*[random_trial(14, 128, 50 + i) for i in range(4)])
+    w = np.array([1.0, 0.8, 1.2, 0.9])
+    orc.set_walkers(list(vds), list(datas), w, np.ones(4), np.zeros(4))
+    eng = Engine(s, tvd, tdata, n_desired=4, capacity=16, max_bond=128, precision="fp64")
+    assert eng.path == "general"
+    eng.load_walkers(list(vds), list(datas), w, np.ones(4), np.zeros(4))
+    E_e = eng.prepare()
+    E_o = orc.energy(1, False)[0]
+    assert abs(E_e - E_o) <= FP64_TOL * abs(E_o)
+    u = Random(21).uniforms(4 * orc.nC)
+    sw_o, idx_o = orc.propagate(u)
+    sw_e = eng.propagate(u)
+    assert np.array_equal(idx_o, eng.indices(4))
+    assert abs(sw_e - sw_o) <= FP64_TOL * abs(sw_o)
+    wo, Oo, _ = orc.scalars()
+    sc = eng.scalars()
+    assert rel(sc["w"], wo) < FP64_TOL and rel(sc["O"], Oo) < FP64_TOL
+    eng.close()
