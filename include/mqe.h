@@ -220,7 +220,8 @@ int mqe_train_proposal(mqe_engine *e, mqe_mlp *m, int epochs, int batch, double 
  * n_out = MQE_NLABEL and stay alive while it is in use. */
 int mqe_set_proposal(mqe_engine *e, mqe_mlp *m, double mix);
 /* Fixed predictions instead of a network: n_walkers * nC * 16 floats in walker
- * order (NULL: the reference's proposal). */
+ * order (NULL: the reference's proposal); walkers beyond n_walkers get the
+ * uniform proposal. */
 int mqe_set_proposal_table(mqe_engine *e, const float *pred, int n_walkers, double mix);
 /* Local energies of the bonds (couplings) from the last energy sweep, n * nC
  * doubles in walker order; they sum to each walker's E_L. */

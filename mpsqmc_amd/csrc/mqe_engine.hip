@@ -2012,7 +2012,10 @@ int mqe_set_proposal_table(mqe_engine *e, const float *pred, int n_walkers, doub
     }
     if (n_walkers < 0 || n_walkers > e->p.capacity) { set_error("mqe_set_proposal_table: bad walker count"); return MQE_EINVAL; }
     ECHK(ensure_pred(e));
-    HIPCHK(hipMemcpy(e->d_pred, pred, sizeof(float) * (size_t)n_walkers * e->nC * MQE_NLABEL, hipMemcpyHostToDevice));
+    // rows beyond the table predict 0: their proposal is the uniform one
+    const size_t rows = (size_t)e->p.capacity * e->nC, given = (size_t)n_walkers * e->nC;
+    HIPCHK(hipMemset(e->d_pred, 0, sizeof(float) * rows * MQE_NLABEL));
+    HIPCHK(hipMemcpy(e->d_pred, pred, sizeof(float) * given * MQE_NLABEL, hipMemcpyHostToDevice));
     e->learn = true;
     e->mix = mix;
     return MQE_OK;

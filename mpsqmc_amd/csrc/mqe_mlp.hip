@@ -297,7 +297,12 @@ int mlp_io_dims(const mqe_mlp *m, int *nin, int *nout) {
 int mlp_fit_device(mqe_mlp *m, const float *dX, const float *dY, long n, int epochs, int batch, double lr,
                    int standardize, double *loss_out, hipStream_t s) {
     if (n < 1 || epochs < 0 || batch < 1) { set_error("mqe_mlp_fit: bad sizes"); return MQE_EINVAL; }
-    hipStream_t keep = m->stream;
+    // the passes run on the caller's stream; the network's own stream is restored on every exit
+    struct StreamSwap {
+        mqe_mlp *m;
+        hipStream_t keep;
+        ~StreamSwap() { m->stream = keep; }
+    } swap{m, m->stream};
     m->stream = s;
     if (standardize) {
         hipLaunchKernelGGL(mlp::stats_kernel, dim3(m->nin), dim3(MQE_NT), 0, s, dX, n, m->nin, m->d_scale,
@@ -323,7 +328,7 @@ int mlp_fit_device(mqe_mlp *m, const float *dX, const float *dY, long n, int epo
         for (long r0 = 0; r0 < n; r0 += batch) {
             const long nb = std::min((long)batch, n - r0);
             int r = grad_pass(m, dX, dY, nb, m->d_perm + r0);
-            if (r) { m->stream = keep; return r; }
+            if (r) return r;
             ++m->adam_t;
             const float c1 = 1.f - powf(b1, (float)m->adam_t), c2 = 1.f - powf(b2, (float)m->adam_t);
             const int tiles = (int)((nb + mlp::TS - 1) / mlp::TS);
@@ -339,7 +344,6 @@ int mlp_fit_device(mqe_mlp *m, const float *dX, const float *dY, long n, int epo
         last = lsum / ((double)n * m->nout);
     }
     if (loss_out) *loss_out = last;
-    m->stream = keep;
     return MQE_OK;
 }
 
