@@ -522,7 +522,7 @@ __device__ void apply_field_everywhere(const Ctx<T> &C) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(MQE_NT) void walker_kernel(Sys S, Pool P, const T *trial, int flags) {
+__global__ __launch_bounds__(MQE_NT, 2) void walker_kernel(Sys S, Pool P, const T *trial, int flags) {
     const int b = blockIdx.x;
     const int n = *P.n;
     if (b >= n) return;
@@ -1391,10 +1391,10 @@ int mqe_create(const mqe_params *p, const mqe_mps *trial, int n_couplings, const
         CK(hipMemset(e->d_plab, 0, sizeof(float) * (size_t)p->capacity * std::max(n_couplings, 1) * 16));
     }
     if (!e->tile32) {
-        // the general kernel runs one workgroup per CU (its register use); its
-        // LQ workspace (Dw tau, then up to 2 Dw x Dw matrix columns) goes to LDS,
-        // as much as 144 KB holds
-        const size_t need = std::min(sizeof(double) * ((size_t)2 * e->Dw * e->Dw + e->Dw), (size_t)144 * 1024);
+        // the general kernel runs two workgroups per CU (launch bounds); its LQ
+        // workspace (Dw tau, then up to 2 Dw x Dw matrix columns) goes to LDS, as
+        // much as 72 KB holds
+        const size_t need = std::min(sizeof(double) * ((size_t)2 * e->Dw * e->Dw + e->Dw), (size_t)72 * 1024);
         if (need >= sizeof(double) * (size_t)(e->Dw + 2 * e->Dw)) {
             e->gen_lds_bytes = need;
             CK(hipFuncSetAttribute((const void *)walker_kernel<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
