@@ -75,6 +75,7 @@ struct Pool {
     const double *u;       // host-supplied uniforms (n*nC, walker-major) or nullptr
     long wflat, eflat;
     const long *woff, *eoff;
+    const long *moff;      // general path: right norm environments M_k at Renv + moff[k] (w_k x w_k)
     unsigned long long seed;
     int step;
     int *status;
@@ -143,7 +144,7 @@ struct Ctx {
     const Sys *S;
     const T *trial;
     T *phi, *Lenv, *Renv;
-    const long *woff, *eoff;
+    const long *woff, *eoff, *moff;
     const int *tv, *wv;  // LDS copies of bond dims
     ScratchLayout sl;
     T *ts;               // T scratch
@@ -156,6 +157,7 @@ struct Ctx {
     __device__ const T *tsite(int k) const { return trial + S->toff[k]; }
     __device__ T *Lk(int k) const { return Lenv + eoff[k]; }
     __device__ T *Rk(int k) const { return Renv + eoff[k]; }
+    __device__ T *Mk(int k) const { return Renv + moff[k]; }  // right norm environment (w_k x w_k)
 };
 
 // Y = L_i phi_i (t_i x 2 w_{i+1});  G = T_i^T Y (2 t_{i+1} x 2 w_{i+1}), block
@@ -318,6 +320,40 @@ __device__ double norm2_at(const Ctx<T> &C, int k, const T *N) {
     return block_sum(s, C.red);
 }
 
+// Right norm environment M_k = sum_s phi_k[s] M_{k+1} phi_k[s]^T (w_k x w_k):
+// the norm of the walker's part right of bond k in the current gauge.
+template <typename T>
+__device__ void advance_M(const Ctx<T> &C, int k) {
+    const int wk = C.wv[k], wn = C.wv[k + 1];
+    T *X = C.ts + C.sl.Y;
+    const T *ph = C.site(k), *Mn = C.Mk(k + 1);
+    T *Mk = C.Mk(k);
+    for (int s = 0; s < 2; ++s)
+        gemm<false, false, T>(wk, wn, wn, ph + (size_t)s * wk * wn, wk, Mn, wn, X + (size_t)s * wk * wn, wk, false);
+    __syncthreads();
+    gemm<false, true, T>(wk, wk, wn, X, wk, ph, wk, Mk, wk, false);
+    __syncthreads();
+    gemm<false, true, T>(wk, wk, wn, X + (size_t)wk * wn, wk, ph + (size_t)wk * wn, wk, Mk, wk, true);
+    __syncthreads();
+}
+
+// Squared norm with the left norm environment N (bond k) and the right norm
+// environment M_{k+1}: sum_s tr(phi_k[s]^T N phi_k[s] M_{k+1}).
+template <typename T>
+__device__ double norm2_at_M(const Ctx<T> &C, int k, const T *N) {
+    const int wk = C.wv[k], wn = C.wv[k + 1];
+    T *Z = C.ts + C.sl.Y, *Z2 = C.ts + C.sl.TMP;
+    const T *ph = C.site(k), *Mn = C.Mk(k + 1);
+    gemm<false, false, T>(wk, 2 * wn, wk, N, wk, ph, wk, Z, wk, false);
+    __syncthreads();
+    for (int s = 0; s < 2; ++s)
+        gemm<false, false, T>(wk, wn, wn, Z + (size_t)s * wk * wn, wk, Mn, wn, Z2 + (size_t)s * wk * wn, wk, false);
+    __syncthreads();
+    double acc = 0.0;
+    for (int e = threadIdx.x; e < wk * 2 * wn; e += MQE_NT) acc += (double)ph[e] * (double)Z2[e];
+    return block_sum(acc, C.red);
+}
+
 template <typename T>
 __device__ void scale_site(const Ctx<T> &C, int k, double f) {
     T *ph = C.site(k);
@@ -385,6 +421,8 @@ __device__ void lq_site(const Ctx<T> &C, int k) {
     const T *V = C.ts + C.sl.V;
     T *Rk = C.Rk(k);
     for (int e = threadIdx.x; e < tk * wk; e += MQE_NT) Rk[e] = V[e] + V[(size_t)tk * 2 * wk + (size_t)tk * wk + e];
+    T *Mk = C.Mk(k);  // sites >= k right-orthonormal
+    for (int e = threadIdx.x; e < wk * wk; e += MQE_NT) Mk[e] = (e % wk == e / wk) ? (T)1 : (T)0;
     __syncthreads();
 }
 
@@ -416,6 +454,7 @@ template <typename T>
 __device__ double right_canonicalize(const Ctx<T> &C) {
     const int L = C.S->L;
     set_one(C.Rk(L));
+    set_one(C.Mk(L));
     for (int k = L - 1; k >= 1; --k) lq_site(C, k);
     double ov, szv;
     site0_overlap(C, &ov, &szv);
@@ -443,6 +482,7 @@ __device__ double energy_sweep(const Ctx<T> &C, double *O_out, double *Ou) {
     const int L = S.L;
     __shared__ double M[16];
     set_one(C.Rk(L));
+    set_one(C.Mk(L));
     double etot = 0.0;
     for (int k = L - 1; k >= 1; --k) {
         lq_site(C, k);  // V of bond k now holds T_k R_{k+1} Q_k^T blocks
@@ -549,6 +589,7 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker_kernel(Sys S, Pool P, const 
     C.Renv = (T *)P.Renv + (size_t)slot * P.eflat;
     C.woff = P.woff;
     C.eoff = P.eoff;
+    C.moff = P.moff;
     C.tv = tv;
     C.wv = wv;
     C.sl = scratch_layout(S.Dt, S.Dw, S.Ng);
@@ -681,13 +722,24 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker_kernel(Sys S, Pool P, const 
             advance_N(C, i, Ncur, Nnxt);
             T *t = Ncur; Ncur = Nnxt; Nnxt = t;
             p = i + 1;
-            nrm2 = norm2_at(C, j, Ncur);
+            nrm2 = norm2_at_M(C, j, Ncur);
             const double nrm = sqrt(nrm2);
             scale_site(C, j, 1.0 / nrm);
             O = 4.0 * fstar / nrm;
         } else {
-            for (int k = j; k >= i + 1; --k) lq_site(C, k);
-            nrm2 = norm2_at(C, i, Ncur);
+            // sites i+1..j are left as they are (no re-orthonormalisation): their
+            // right environments and right norm environments are rebuilt, and the
+            // norm comes from the left norm environment, site i and M_{i+1}
+            for (int k = j; k >= i + 1; --k) {
+                compute_V(C, k);
+                const int tk = tv[k], wk = wv[k];
+                const T *V = C.ts + C.sl.V;
+                T *Rk = C.Rk(k);
+                for (int e = threadIdx.x; e < tk * wk; e += MQE_NT)
+                    Rk[e] = V[e] + V[(size_t)tk * 2 * wk + (size_t)tk * wk + e];
+                advance_M(C, k);
+            }
+            nrm2 = norm2_at_M(C, i, Ncur);
             const double nrm = sqrt(nrm2);
             scale_site(C, i, 1.0 / nrm);
             O = 4.0 * fstar / nrm;
@@ -1076,13 +1128,13 @@ struct mqe_engine {
     float *d_t32N = nullptr, *d_t32T = nullptr;
     hipStream_t stream = nullptr;
     std::vector<int> tvd, wmax;
-    std::vector<long> toff, woff, eoff;
+    std::vector<long> toff, woff, eoff, moff;
     long wflat = 0, eflat = 0, scratch_bytes = 0;
     // device
     void *d_trial = nullptr;
     int *d_tvd = nullptr, *d_first = nullptr, *d_second = nullptr, *d_nextfirst = nullptr, *d_csptr = nullptr,
         *d_csidx = nullptr;
-    long *d_toff = nullptr, *d_woff = nullptr, *d_eoff = nullptr;
+    long *d_toff = nullptr, *d_woff = nullptr, *d_eoff = nullptr, *d_moff = nullptr;
     double *d_Jc = nullptr, *d_S = nullptr, *d_U = nullptr, *d_VT = nullptr, *d_grid = nullptr;
     void *d_phi = nullptr, *d_Renv = nullptr, *d_Lenv = nullptr;
     int *d_wvd = nullptr;
@@ -1141,6 +1193,7 @@ static Pool make_pool(mqe_engine *e, const double *u, int step) {
     P.eflat = e->eflat;
     P.woff = e->d_woff;
     P.eoff = e->d_eoff;
+    P.moff = e->d_moff;
     P.seed = e->p.seed;
     P.step = step;
     P.status = e->d_status;
@@ -1327,6 +1380,11 @@ int mqe_create(const mqe_params *p, const mqe_mps *trial, int n_couplings, const
         e->eoff[k] = eo;
         eo += (long)e->tvd[k] * e->wmax[k];
     }
+    e->moff.resize(L + 1);  // general path: right norm environments after the right environments
+    for (int k = 0; k <= L; ++k) {
+        e->moff[k] = eo;
+        eo += (long)e->wmax[k] * e->wmax[k];
+    }
     e->wflat = (wo + 63) & ~63L;
     e->eflat = (eo + 63) & ~63L;
     const long tsize = to;
@@ -1422,6 +1480,7 @@ int mqe_create(const mqe_params *p, const mqe_mps *trial, int n_couplings, const
     CK(up((void **)&e->d_toff, e->toff.data(), sizeof(long) * L));
     CK(up((void **)&e->d_woff, e->woff.data(), sizeof(long) * L));
     CK(up((void **)&e->d_eoff, e->eoff.data(), sizeof(long) * (L + 1)));
+    CK(up((void **)&e->d_moff, e->moff.data(), sizeof(long) * (L + 1)));
     CK(up((void **)&e->d_first, first, sizeof(int) * nC));
     CK(up((void **)&e->d_second, second, sizeof(int) * nC));
     CK(up((void **)&e->d_nextfirst, nextfirst.data(), sizeof(int) * nC));
@@ -1518,7 +1577,7 @@ void mqe_destroy(mqe_engine *e) {
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
     void *ptrs[] = {e->d_t32N, e->d_t32T, e->d_trial, e->d_tvd, e->d_first, e->d_second, e->d_nextfirst, e->d_csptr, e->d_csidx,
-                    e->d_toff, e->d_woff, e->d_eoff, e->d_Jc, e->d_S, e->d_U, e->d_VT, e->d_grid, e->d_phi,
+                    e->d_toff, e->d_woff, e->d_eoff, e->d_moff, e->d_Jc, e->d_S, e->d_U, e->d_VT, e->d_grid, e->d_phi,
                     e->d_Renv, e->d_Lenv, e->d_wvd, e->d_w, e->d_O, e->d_eh, e->d_EL, e->d_order, e->d_order2,
                     e->d_n, e->d_idx, e->d_u, e->d_scratch, e->d_status, e->d_sums, e->d_trace, e->d_ncopy,
                     e->d_newpos, e->d_used, e->d_free, e->d_csrc, e->d_cdst, e->d_nextra, e->d_nnew, e->d_nused,
