@@ -477,7 +477,7 @@ __device__ double right_canonicalize(const Ctx<T> &C) {
 // Returns e_tot = <trial|H|phi> of the state as it was on entry and writes the
 // overlap of the normalised state to *O_out, <trial|phi> before normalisation to *Ou.
 template <typename T>
-__device__ double energy_sweep(const Ctx<T> &C, double *O_out, double *Ou) {
+__device__ double energy_sweep(const Ctx<T> &C, double *O_out, double *Ou, bool canon) {
     const Sys &S = *C.S;
     const int L = S.L;
     __shared__ double M[16];
@@ -485,7 +485,18 @@ __device__ double energy_sweep(const Ctx<T> &C, double *O_out, double *Ou) {
     set_one(C.Mk(L));
     double etot = 0.0;
     for (int k = L - 1; k >= 1; --k) {
-        lq_site(C, k);  // V of bond k now holds T_k R_{k+1} Q_k^T blocks
+        if (canon) {
+            lq_site(C, k);  // V of bond k now holds T_k R_{k+1} Q_k^T blocks
+        } else {
+            // light sweep: the gauge stays; R_k and the right norm environment M_k
+            compute_V(C, k);
+            const int tk = C.tv[k], wk = C.wv[k];
+            const T *V = C.ts + C.sl.V;
+            T *Rk = C.Rk(k);
+            for (int e = threadIdx.x; e < tk * wk; e += MQE_NT)
+                Rk[e] = V[e] + V[(size_t)tk * 2 * wk + (size_t)tk * wk + e];
+            advance_M(C, k);
+        }
         for (int q = S.cs_ptr[k]; q < S.cs_ptr[k + 1]; ++q) {
             const int c = S.cs_idx[q], i = S.first[c];
             compute_G(C, i, C.Lk(i));
@@ -540,11 +551,13 @@ __device__ double energy_sweep(const Ctx<T> &C, double *O_out, double *Ou) {
     site0_overlap(C, &ov, &szv);
     if (S.h != 0.0) etot -= S.h * szv;
     double nrm2;
-    {
+    if (canon) {
         const T *p0 = C.site(0);
         double s = 0.0;
         for (int e = threadIdx.x; e < 2 * C.wv[1]; e += MQE_NT) s += (double)p0[e] * (double)p0[e];
         nrm2 = block_sum(s, C.red);
+    } else {
+        nrm2 = norm2_at_M(C, 0, C.Lk(0));  // L_0 = 1 is also the left norm environment of site 0
     }
     const double nrm = sqrt(nrm2);
     double O = ov / nrm, f = 1.0 / nrm;
@@ -610,7 +623,7 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker_kernel(Sys S, Pool P, const 
     if (flags & F_PREP) {
         for (int k = 0; k < L - 1; ++k) advance_L(C, k);
         double On, Ou;
-        const double e = energy_sweep(C, &On, &Ou);
+        const double e = energy_sweep(C, &On, &Ou, true);
         if (threadIdx.x == 0) {
             P.EL[slot] = e / O;  // stored overlap, MPSQMC2.cpp:447 / Walker.cpp:72-77
             P.O[slot] = On;
@@ -764,7 +777,7 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker_kernel(Sys S, Pool P, const 
     }
     // ---------------- local energy (Walker.cpp:72-77) ----------------
     double On, Ou;
-    const double e = energy_sweep(C, &On, &Ou);
+    const double e = energy_sweep(C, &On, &Ou, (flags & F_CANON) != 0);
     if (threadIdx.x == 0) {
         P.w[slot] = w;
         P.O[slot] = On;
