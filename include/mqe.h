@@ -114,10 +114,10 @@ int mqe_read_trace(mqe_engine *e, int first_step, int n_steps, double *out);
  * all bonds <= 32, h = 0), 0 for the general path. */
 int mqe_kernel_path(mqe_engine *e);
 
-/* Tile path: re-orthonormalise the walkers (FP64 Householder LQ sweep) every
- * `steps` propagations (default 64); in between the energy sweep carries right
- * norm environments instead, which leaves every computed quantity unchanged
- * and changes only the gauge of the stored tensors.  1 = every step. */
+/* Re-orthonormalise the walkers (FP64 Householder LQ sweep) every `steps`
+ * propagations (default 64); in between the energy sweep carries right norm
+ * environments instead, which leaves every computed quantity unchanged and
+ * changes only the gauge of the stored tensors.  1 = every step. */
 int mqe_set_canonical_interval(mqe_engine *e, int steps);
 
 int mqe_sync(mqe_engine *e);
