@@ -488,7 +488,21 @@ __device__ double energy_sweep(const Ctx<T> &C, double *O_out, double *Ou, bool 
         if (canon) {
             lq_site(C, k);  // V of bond k now holds T_k R_{k+1} Q_k^T blocks
         } else {
-            // light sweep: the gauge stays; R_k and the right norm environment M_k
+            // light sweep: no re-orthonormalisation; site k is rescaled to unit
+            // Frobenius norm and the factor moved into site k-1 (the state is
+            // unchanged; this keeps the tensors' magnitudes from drifting apart),
+            // then R_k and the right norm environment M_k
+            {
+                const int wk = C.wv[k], wn = C.wv[k + 1];
+                const T *ph = C.site(k);
+                double s2 = 0.0;
+                for (int e = threadIdx.x; e < 2 * wk * wn; e += MQE_NT) s2 += (double)ph[e] * (double)ph[e];
+                const double f = sqrt(block_sum(s2, C.red));
+                if (f > 0.0 && isfinite(f)) {
+                    scale_site(C, k, 1.0 / f);
+                    scale_site(C, k - 1, f);
+                }
+            }
             compute_V(C, k);
             const int tk = C.tv[k], wk = C.wv[k];
             const T *V = C.ts + C.sl.V;
