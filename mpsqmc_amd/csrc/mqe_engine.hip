@@ -488,16 +488,19 @@ __device__ double energy_sweep(const Ctx<T> &C, double *O_out, double *Ou, bool 
         if (canon) {
             lq_site(C, k);  // V of bond k now holds T_k R_{k+1} Q_k^T blocks
         } else {
-            // light sweep: no re-orthonormalisation; site k is rescaled to unit
-            // Frobenius norm and the factor moved into site k-1 (the state is
-            // unchanged; this keeps the tensors' magnitudes from drifting apart),
-            // then R_k and the right norm environment M_k
+            // light sweep: no re-orthonormalisation; site k is rescaled to the
+            // Frobenius norm sqrt(w_k) of a right-canonical site and the factor
+            // moved into site k-1 (the state is unchanged).  This keeps every
+            // tensor, and the environments built from them, at the magnitudes of
+            // the canonical form: a unit-norm target would shrink R_k by about
+            // sqrt(w) per site and underflow FP32 on long chains at large D.
+            // Then R_k and the right norm environment M_k.
             {
                 const int wk = C.wv[k], wn = C.wv[k + 1];
                 const T *ph = C.site(k);
                 double s2 = 0.0;
                 for (int e = threadIdx.x; e < 2 * wk * wn; e += MQE_NT) s2 += (double)ph[e] * (double)ph[e];
-                const double f = sqrt(block_sum(s2, C.red));
+                const double f = sqrt(block_sum(s2, C.red) / (double)wk);
                 if (f > 0.0 && isfinite(f)) {
                     scale_site(C, k, 1.0 / f);
                     scale_site(C, k - 1, f);

@@ -311,3 +311,41 @@ def test_parity_large_bond_fp64():
     sc = eng.scalars()
     assert rel(sc["w"], wo) < FP64_TOL and rel(sc["O"], Oo) < FP64_TOL
     eng.close()
+
+
+def test_light_sweep_long_chain_d128_fp32():
+    # steps between re-orthonormalisations rescale each site to the magnitude of a
+    # right-canonical one; 36 sites at D = 128 with walkers nearly orthogonal to
+    # the trial (O ~ 1e-6) are where a wrong target scale underflows FP32.  The
+    # light steps must give the weights, overlaps and local energies of
+    # re-orthonormalising every step (same walkers, same uniforms).
+    s = systems.chain(36)
+    tvd, tdata = random_trial(36, 128, 1)
+    vds, datas = zip(*[random_trial(36, 128, 40 + i) for i in range(4)])
+    runs = []
+    for canon in (64, 1):
+        eng = Engine(s, tvd, tdata, n_desired=4, capacity=16, max_bond=128, precision="fp32", seed=3)
+        assert eng.path == "general"
+        eng.set_canonical_interval(canon)
+        eng.load_walkers(list(vds), list(datas), np.ones(4), np.ones(4), np.zeros(4))
+        eng.prepare()
+        rng = Random(9)
+        out = []
+        for step in range(3):
+            eng.propagate(rng.uniforms(4 * eng.nC))
+            sc = eng.scalars()
+            out.append((eng.indices(4).copy(), sc["w"].copy(), sc["O"].copy(), sc["EL"].copy()))
+        eng.close()
+        runs.append(out)
+    for (i1, w1, O1, E1), (i2, w2, O2, E2) in zip(*runs):
+        assert np.all(np.isfinite(w1)) and np.all(w1 > 0) and np.all(np.isfinite(E1))
+        same = np.all(i1 == i2, axis=1)
+        if not same.any():
+            break
+        # FP32 conditioning of overlap-derived quantities, as in the parity tests
+        tol = 2e-4 * np.maximum(1.0, 1e-3 / np.abs(O2[same]))
+        assert np.all(np.abs(w1[same] - w2[same]) <= tol * np.abs(w2[same]))
+        assert np.all(np.abs(O1[same] - O2[same]) <= tol * np.abs(O2[same]))
+        assert np.all(np.abs(E1[same] - E2[same]) <= 10 * tol * np.maximum(np.abs(E2[same]), 1.0))
+        if not same.all():
+            break
