@@ -144,8 +144,12 @@ __device__ __forceinline__ void block_inclusive_scan(int n, const double *x, dou
 // FP32 GEMM on the matrix cores: C(m x n) (+)= op(A) op(B), whole workgroup.
 // The output is cut into 32 x 32 tiles, one wave per tile at a time; each tile
 // is a chain of v_mfma_f32_32x32x2f32 over k with operands read straight from
-// (L2-resident) memory, zero outside the matrices.  Lane l holds A(l % 32, kk)
-// and B(kk, l % 32) for kk = k0 + l / 32; accumulator slot q of lane l is
+// (L2-resident) memory, zero outside the matrices.  Within a 16-deep slice of
+// k starting at k0, MFMA u of the chain takes kk = k0 + 8(l/32) + u in lane l:
+// lane l holds A(l % 32, kk) and B(kk, l % 32), so each lane's eight values of
+// an operand stored contiguously along k are one 32-byte run (two 16-byte
+// loads when aligned), and an operand stored contiguously along its other
+// index is read coalesced across the lanes.  Accumulator slot q of lane l is
 // element (8(q/4) + 4(l/32) + q%4, l % 32) of the tile.
 // ---------------------------------------------------------------------------
 template <bool TA, bool TB>
@@ -155,6 +159,9 @@ __device__ __forceinline__ void gemm_mfma(int m, int n, int k, const float *A, i
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int r = lane & 31, h = lane >> 5;
     const int tm = (m + 31) >> 5, tn = (n + 31) >> 5;
+    // 16-byte loads along k need 16-byte aligned bases and leading dimensions
+    const bool avec = TA && ((((uintptr_t)A) | ((uintptr_t)lda * 4u)) & 15u) == 0;
+    const bool bvec = !TB && ((((uintptr_t)B) | ((uintptr_t)ldb * 4u)) & 15u) == 0;
     for (int t = w; t < tm * tn; t += MQE_WAVES) {
         const int r0 = (t % tm) * 32, c0 = (t / tm) * 32;
         const int ar = r0 + r, bc = c0 + r;
@@ -163,13 +170,32 @@ __device__ __forceinline__ void gemm_mfma(int m, int n, int k, const float *A, i
 #pragma unroll
         for (int q = 0; q < 16; ++q) c[q] = 0.f;
         for (int k0 = 0; k0 < k; k0 += 16) {
+            const int kb = k0 + 8 * h;
+            const bool full = k0 + 16 <= k;
             float a[8], b[8];
+            if (avec && full) {
+                const float4 *p = (const float4 *)(A + (size_t)lda * (aok ? ar : 0) + kb);
+                const float4 x = p[0], y = p[1];
+                a[0] = aok ? x.x : 0.f; a[1] = aok ? x.y : 0.f; a[2] = aok ? x.z : 0.f; a[3] = aok ? x.w : 0.f;
+                a[4] = aok ? y.x : 0.f; a[5] = aok ? y.y : 0.f; a[6] = aok ? y.z : 0.f; a[7] = aok ? y.w : 0.f;
+            } else {
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int kk = k0 + 2 * u + h;
-                const bool kok = kk < k;
-                a[u] = (aok && kok) ? (TA ? A[kk + (size_t)lda * ar] : A[ar + (size_t)lda * kk]) : 0.f;
-                b[u] = (bok && kok) ? (TB ? B[bc + (size_t)ldb * kk] : B[kk + (size_t)ldb * bc]) : 0.f;
+                for (int u = 0; u < 8; ++u) {
+                    const int kk = kb + u;
+                    a[u] = (aok && kk < k) ? (TA ? A[kk + (size_t)lda * ar] : A[ar + (size_t)lda * kk]) : 0.f;
+                }
+            }
+            if (bvec && full) {
+                const float4 *p = (const float4 *)(B + (size_t)ldb * (bok ? bc : 0) + kb);
+                const float4 x = p[0], y = p[1];
+                b[0] = bok ? x.x : 0.f; b[1] = bok ? x.y : 0.f; b[2] = bok ? x.z : 0.f; b[3] = bok ? x.w : 0.f;
+                b[4] = bok ? y.x : 0.f; b[5] = bok ? y.y : 0.f; b[6] = bok ? y.z : 0.f; b[7] = bok ? y.w : 0.f;
+            } else {
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int kk = kb + u;
+                    b[u] = (bok && kk < k) ? (TB ? B[bc + (size_t)ldb * kk] : B[kk + (size_t)ldb * bc]) : 0.f;
+                }
             }
 #pragma unroll
             for (int u = 0; u < 8; ++u) c = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u], b[u], c, 0, 0, 0);
