@@ -1406,14 +1406,16 @@ int mqe_create(const mqe_params *p, const mqe_mps *trial, int n_couplings, const
         e->woff[k] = wo;
         wo += (long)e->wmax[k] * e->wmax[k + 1] * 2;
     }
+    // every environment starts on a 4-element boundary, so that the matrix-core
+    // products can read it with 16-byte loads
     for (int k = 0; k <= L; ++k) {
         e->eoff[k] = eo;
-        eo += (long)e->tvd[k] * e->wmax[k];
+        eo += ((long)e->tvd[k] * e->wmax[k] + 3) & ~3L;
     }
     e->moff.resize(L + 1);  // general path: right norm environments after the right environments
     for (int k = 0; k <= L; ++k) {
         e->moff[k] = eo;
-        eo += (long)e->wmax[k] * e->wmax[k];
+        eo += ((long)e->wmax[k] * e->wmax[k] + 3) & ~3L;
     }
     e->wflat = (wo + 63) & ~63L;
     e->eflat = (eo + 63) & ~63L;
