@@ -33,6 +33,17 @@ F64_PER_WS = 15.9e6
 PEAK_FP32 = 157.3e12   # MI355X dense FP32 matrix
 PEAK_FP64 = 78.6e12    # MI355X FP64
 
+# Open-boundary snake lattices on the general path (long-range couplings):
+# trial bond D_T, walker bonds drawn uniformly from D_W (the reference runs one
+# D_W per job), and the environment-sweep FP32 / FP64 flops per walker-step
+# for each walker bond.
+SNAKES = {
+    "snake4x4": dict(rows=4, cols=4, DT=64, DW=(16, 32, 48, 64), N=4096,
+                     F32={16: 84.5e6, 32: 200e6, 48: 325e6, 64: 474e6},
+                     F64={16: 1.8e6, 32: 11e6, 48: 29e6, 64: 62e6}),
+    "snake6x6": dict(rows=6, cols=6, DT=128, DW=(128,), N=1024, F32={128: 25.7e9}, F64={128: 3.0e9}),
+}
+
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
@@ -41,8 +52,11 @@ def log(msg):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2048)
-    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--workload", default="chain", choices=["chain"] + sorted(SNAKES),
+                    help="chain: the L=32, D=32 chain (default); snake4x4 / snake6x6: the 2D snake lattices "
+                         "on the general path")
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (chain 2048, snakes 64)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps (chain 200, snakes 1)")
     ap.add_argument("--walkers", type=int, default=2048)
     ap.add_argument("--L", type=int, default=32)
     ap.add_argument("--D", type=int, default=32)
@@ -59,7 +73,13 @@ def parse():
                     help="weight of the uniform part of the learned proposal (keeps its support)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB ten times (reference for HBM counters)")
-    return ap.parse_args()
+    args = ap.parse_args()
+    chain = args.workload == "chain"
+    if args.steps is None:
+        args.steps = 2048 if chain else 64
+    if args.warmup is None:
+        args.warmup = 200 if chain else 1
+    return args
 
 
 def calibrate_copy(mib):
@@ -165,6 +185,77 @@ def learned_comparison(eng, args, first, plain_vt):
             "gain_rel_error": float(np.hypot(plain_vt["var_rel_error"], vt["var_rel_error"])) if plain_vt else None}
 
 
+def snake_workload(args):
+    """Walker-steps/s of one snake lattice on the general path: random
+    left-normalised trial and walker MPS of the stated bonds (throughput does
+    not depend on the tensor values), device RNG, population control."""
+    from mpsqmc_amd import mps as M
+    from mpsqmc_amd import systems
+    from mpsqmc_amd.engine import Engine
+
+    w = SNAKES[args.workload]
+    system = systems.snake_lattice(w["rows"], w["cols"])
+    L, N = system.L, w["N"]
+    tvd, tdata = M.random_mps(L, w["DT"], 2, 1)
+    tdata, _ = M.left_normalize(tvd, tdata)
+    rng = np.random.default_rng(2)
+    states, vds, datas, counts = {}, [], [], {D: 0 for D in w["DW"]}
+    for _ in range(N):
+        D = int(rng.choice(w["DW"]))
+        if D not in states:
+            vd, data = M.random_mps(L, D, 2, 10 + D)
+            states[D] = (vd, M.left_normalize(vd, data)[0])
+        vds.append(states[D][0])
+        datas.append(states[D][1])
+        counts[D] += 1
+    eng = Engine(system, tvd, tdata, n_desired=N, capacity=3 * N, max_bond=max(w["DW"]), precision=args.precision,
+                 seed=args.seed)
+    eng.load_walkers(vds, datas)
+    eng.prepare()
+    log(f"bench: {args.workload} engine ready ({N} walkers, L={L}, path {eng.path}); warmup {args.warmup} steps")
+    eng.run(1, args.warmup)
+    eng.sync()
+    eng.set_timing(True)
+    first = args.warmup + 1
+    t0 = time.perf_counter()
+    for c in range(0, args.steps, 16):  # progress lines: a D = 128 step takes over a second
+        eng.run(first + c, min(16, args.steps - c))
+        eng.sync()
+        log(f"bench: {min(c + 16, args.steps)} steps, {time.perf_counter() - t0:.1f} s")
+    wall = time.perf_counter() - t0
+    ktot_ms, nlaunch = eng.kernel_time()
+    trace = eng.read_trace(first, args.steps)
+    eng.close()
+    walker_steps = float(trace[:, 1].sum())
+    value = walker_steps / wall
+    # flops per walker-step of the initial bond mixture (population control
+    # copies walkers by weight, which can shift the mixture a little)
+    f32 = sum(counts[D] * w["F32"][D] for D in counts) / N
+    f64 = sum(counts[D] * w["F64"][D] for D in counts) / N
+    avg_launch_s = ktot_ms / 1000.0 / max(nlaunch, 1)
+    walkers_per_launch = walker_steps / max(nlaunch, 1)
+    achieved = (f32 + f64) * walkers_per_launch / avg_launch_s / 1e12
+    peak = (f32 + f64) / (f32 / PEAK_FP32 + f64 / PEAK_FP64) / 1e12
+    dws = "{" + ",".join(str(D) for D in w["DW"]) + "}"
+    return {
+        "metric": f"walker-steps/s ({w['rows']}x{w['cols']} snake, D_T={w['DT']}, D_W in {dws}, {N} walkers)",
+        "value": value, "unit": "walker-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1000.0, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64" if args.precision == "fp32" else "f64",
+        "data": "synthetic: random left-normalized trial and walker MPS of the stated bond profiles",
+        "config": {"workload": f"heisenberg {w['rows']}x{w['cols']} OBC snake (L={L}), trial D={w['DT']}, "
+                               f"walkers D in {dws}, N={N}, dtau=0.01, Marsaglia(4) grid, "
+                               f"{args.precision} contractions", "walkers": N, "L": L},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                     "traffic": None, "kernel": "walker_kernel", "avg_launch_ms": avg_launch_s * 1000.0,
+                     "flops_per_walker_step": f32 + f64},
+        "cpu_baseline": None,
+        "E_P_last": float(trace[-1, 2]),
+        "population_mean": float(trace[:, 1].mean()),
+        "population_last": int(trace[-1, 1]),
+    }
+
+
 def main():
     args = parse()
     if args.calibrate_copy:
@@ -174,6 +265,9 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               f"(walker sharding over GPUs is designed but not built)", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "chain":
+        print(json.dumps(snake_workload(args)), flush=True)
+        return
     from mpsqmc_amd import systems
     from mpsqmc_amd.engine import Engine
 
