@@ -201,7 +201,8 @@ __device__ void carry_channels(const Ctx<T> &C, int k, const T *src, T *dst) {
         __syncthreads();
         T *d = dst + (size_t)ch * tn * wn;
         gemm<true, false, T>(tn, wn, tk, Tk, tk, Y, tk, d, tn, false);
-        __syncthreads();
+        // no barrier: the accumulating product has the same shape, so every
+        // thread re-reads only the output elements it wrote itself
         gemm<true, false, T>(tn, wn, tk, Tk + (size_t)tk * tn, tk, Y + (size_t)tk * wn, tk, d, tn, true);
         __syncthreads();
     }
@@ -288,7 +289,7 @@ __device__ void advance_L(const Ctx<T> &C, int k) {
     const T *Tk = C.tsite(k);
     T *Ln = C.Lk(k + 1);
     gemm<true, false, T>(tn, wn, tk, Tk, tk, Y, tk, Ln, tn, false);
-    __syncthreads();
+    // same-shape accumulation: no barrier needed (each thread owns its outputs)
     gemm<true, false, T>(tn, wn, tk, Tk + (size_t)tk * tn, tk, Y + (size_t)tk * wn, tk, Ln, tn, true);
     __syncthreads();
 }
@@ -302,7 +303,7 @@ __device__ void advance_N(const Ctx<T> &C, int k, const T *N, T *Nn) {
     gemm<false, false, T>(wk, 2 * wn, wk, N, wk, ph, wk, X, wk, false);
     __syncthreads();
     gemm<true, false, T>(wn, wn, wk, ph, wk, X, wk, Nn, wn, false);
-    __syncthreads();
+    // same-shape accumulation: no barrier needed (each thread owns its outputs)
     gemm<true, false, T>(wn, wn, wk, ph + (size_t)wk * wn, wk, X + (size_t)wk * wn, wk, Nn, wn, true);
     __syncthreads();
 }
@@ -332,7 +333,7 @@ __device__ void advance_M(const Ctx<T> &C, int k) {
         gemm<false, false, T>(wk, wn, wn, ph + (size_t)s * wk * wn, wk, Mn, wn, X + (size_t)s * wk * wn, wk, false);
     __syncthreads();
     gemm<false, true, T>(wk, wk, wn, X, wk, ph, wk, Mk, wk, false);
-    __syncthreads();
+    // same-shape accumulation: no barrier needed (each thread owns its outputs)
     gemm<false, true, T>(wk, wk, wn, X + (size_t)wk * wn, wk, ph + (size_t)wk * wn, wk, Mk, wk, true);
     __syncthreads();
 }
