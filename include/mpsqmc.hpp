@@ -28,9 +28,12 @@ class Random {
     void fill(long n, double *out);
     void peek(long n, double *out);
     void discard(long k);
+    // the seed; MPSQMC2 also keys the device random numbers with it
+    unsigned int seed() const { return seed_; }
 
   private:
     mqe_rng *rng_;
+    unsigned int seed_;
 };
 
 // GridGenerator.h:13-34
@@ -100,10 +103,23 @@ class MPSstate {
     int gDtrunc() const { return Dtrunc_; }
     int gDimAtBound(int bound) const { return (bound < 0 || bound > L_) ? 0 : vd_[bound]; }
     double LeftNormalize();
+    double RightNormalize();
     double InnerProduct(const MPSstate *other) const;
+    void ScalarMultiplication(double factor);
     void ChangePhase();
     void CompressState(int truncD);
     void CompressState() { CompressState(Dtrunc_); }
+    // this <- H Psi0 (MPSstate.cpp:413-471); bond dims become MPO channels x Psi0's
+    void ApplyMPO(const HeisenbergMPO *theMPO, const MPSstate *Psi0);
+    // sqrt(s_a) U_a on firstSite and sqrt(s_b) V_b on secondSite, transposed if
+    // doHC (MPSstate.cpp:505-573)
+    void ApplyTwoSiteTrotterTerm(const TrotterHeisenberg *theTrotter, int firstSite, int secondSite,
+                                 int leftSVDindex, int rightSVDindex, bool doHC = false);
+    // the grid-point operator sum_k sqrt(s_k dim) x_k U_k (x) V_k (MPSstate.cpp:575-633)
+    void ApplyTwoSiteTrotterTerm(const TrotterHeisenberg *theTrotter, int firstSite, int secondSite,
+                                 const GridGenerator *theGrid, int gridPoint);
+    // exp(dtau h S^z / 2) on every site if the field is on (MPSstate.cpp:635-659)
+    void ApplyOneSiteTrotterTermEverywhere(const TrotterHeisenberg *theTrotter);
     const std::vector<int> &bonds() const { return vd_; }
     void assign(const std::vector<int> &vd, const std::vector<double> &data);
     const std::vector<double> &data() const { return data_; }
@@ -112,6 +128,7 @@ class MPSstate {
 
   private:
     long offset(int site) const;
+    void apply_site(int site, const double *op);  // A[s'] <- sum_s op[s' + d s] A[s]
     int L_, Dtrunc_, d_;
     std::vector<int> vd_;
     std::vector<double> data_;
@@ -133,15 +150,29 @@ class DMRG {
     unsigned int seed_;
 };
 
-// Walker.h:13-52 (a host copy of one GPU walker)
+// Walker.h:13-52: one walker on the host (a copy of a GPU walker, or a walker
+// the caller builds and propagates itself with the MPSstate methods)
 class Walker {
   public:
     Walker(const MPSstate *state, double overlap, double weight, double energyHistory)
         : state_(state), overlap_(overlap), weight_(weight), eh_(energyHistory) {}
+    explicit Walker(const Walker *other)
+        : state_(&other->state_), overlap_(other->overlap_), weight_(other->weight_), eh_(other->eh_) {}
+    // random MPSstate, weight 1, overlap NAN, energy history 0 (Walker.cpp:28-35)
+    Walker(int length, int Dtrunc, int phys_d, Random *RN);
     double gWeight() const { return weight_; }
     double gOverlap() const { return overlap_; }
     double gEnergyHistory() const { return eh_; }
+    MPSstate *gState() { return &state_; }
     const MPSstate *gState() const { return &state_; }
+    void setWeight(double weight) { weight_ = weight; }
+    void multiplyWeight(double factor) { weight_ *= factor; }
+    // left-normalise, overlap with the trial, phase fixed so that it is > 0 (Walker.cpp:54-66)
+    void setOverlap(const MPSstate *theTrial);
+    void setEnergyHistory(double energyHistory) { eh_ = energyHistory; }
+    void addEnergyToHistory(double term) { eh_ += term; }
+    // <H Psi_T | walker> / overlap; setOverlap must have been called (Walker.cpp:72-77)
+    double calcIndividualProjectedEnergy(const MPSstate *HamTimesTrial) const;
 
   private:
     MPSstate state_;

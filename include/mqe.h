@@ -80,7 +80,10 @@ int mqe_prepare(mqe_engine *e, double *E_start);
 /* Replaces MPSQMC2::PropagateSeparately (MPSQMC2.cpp:518-611) and computes the
  * per-walker local energies of the propagated walkers (Walker.cpp:72-77).
  * uniforms: n_walkers*n_couplings draws, walker-major, or NULL for the device
- * RNG.  *sum_w receives the sum of the walker weights. */
+ * RNG: the k-th call with NULL draws the numbers of step k of mqe_run (Philox
+ * keyed by seed, step word k, population position and coupling), so a walk of
+ * mqe_propagate / mqe_energy / mqe_population_control calls with device draws
+ * repeats mqe_run(1, ...) exactly.  *sum_w receives the sum of the weights. */
 int mqe_propagate(mqe_engine *e, const double *uniforms, double *sum_w);
 
 /* Replaces MPSQMC2::EnergyFunctionAndHistory (MPSQMC2.cpp:688-747): E_P and
@@ -89,8 +92,12 @@ int mqe_energy(mqe_engine *e, int step, int do_fluct, double *E_P, double *omega
 
 /* Replaces MPSQMC2::SeparatePopulationControl (MPSQMC2.cpp:613-686).  The
  * k-th walker (in population order) whose scaled weight lies outside
- * [0.25, 1.5] consumes uniforms[k]; uniforms may be NULL for the device RNG.
- * *n_new receives the new population size, *n_used the draws consumed. */
+ * [0.25, 1.5] consumes uniforms[k]; uniforms may be NULL for the device RNG
+ * (the k-th such call draws the numbers of step k of mqe_run).  *n_new receives
+ * the new population size, *n_used the draws consumed.  MQE_ECAPACITY: the
+ * copies would exceed the capacity; the population is left unchanged and the
+ * engine refuses further steps (the reference writes past its arrays,
+ * MPSQMC2.cpp:650-662). */
 int mqe_population_control(mqe_engine *e, double scaling, const double *uniforms, int n_uniforms,
                            int *n_new, int *n_used);
 
@@ -100,11 +107,12 @@ int mqe_population_control(mqe_engine *e, double scaling, const double *uniforms
  * Omega, sum_w) is kept on the device and read with mqe_read_trace. */
 int mqe_step(mqe_engine *e, int step);
 
-/* Run steps first_step .. first_step+n_steps-1 with mqe_step.  use_graph is
- * accepted and ignored: the launches of a step carry its number (the device
- * RNG key) and the re-orthonormalisation flag as arguments, and the host
- * enqueues them asynchronously far ahead of the GPU (~8 launches per
- * multi-millisecond step), so a replayed graph would save no GPU time. */
+/* Run steps first_step .. first_step+n_steps-1 with mqe_step.  use_graph != 0:
+ * the n_steps steps are captured once in a HIP graph (kept until a run with
+ * another n_steps) and replayed; step numbers live in a device control block
+ * that the graph advances, so a replay gives exactly the eager result.  Graph
+ * replay needs the reference proposal without sample collection (MQE_EINVAL
+ * otherwise) and is not timed by mqe_set_timing. */
 int mqe_run(mqe_engine *e, int first_step, int n_steps, int use_graph);
 
 /* Copy the trace rows of steps first_step .. first_step+n_steps-1 (6 doubles each). */
@@ -114,8 +122,9 @@ int mqe_read_trace(mqe_engine *e, int first_step, int n_steps, double *out);
  * all bonds <= 32, h = 0), 0 for the general path. */
 int mqe_kernel_path(mqe_engine *e);
 
-/* Re-orthonormalise the walkers (FP64 Householder LQ sweep) every `steps`
- * propagations (default 64); in between the energy sweep carries right norm
+/* Re-orthonormalise every walker (FP64 Householder LQ sweep) once in `steps`
+ * propagations (default 64), staggered over the walker slots so that every
+ * step does the same share; in between the energy sweep carries right norm
  * environments instead, which leaves every computed quantity unchanged and
  * changes only the gauge of the stored tensors.  1 = every step. */
 int mqe_set_canonical_interval(mqe_engine *e, int steps);
@@ -127,7 +136,8 @@ int mqe_num_couplings(mqe_engine *e, int *n);
 /* Walker scalars in population order (any pointer may be NULL). */
 int mqe_read_scalars(mqe_engine *e, double *w, double *O, double *eh, double *EL);
 
-/* Sampled grid point of every (walker, coupling) of the last mqe_propagate, walker-major. */
+/* Sampled grid point of every (walker, coupling) of the last propagation,
+ * walker-major; idx holds (its population, see mqe_last_kernel_ms) x nC ints. */
 int mqe_read_indices(mqe_engine *e, int *idx);
 
 /* Walker i (population order): bond dims and tensors (gauge is the engine's,
@@ -138,8 +148,9 @@ int mqe_read_walker(mqe_engine *e, int i, int *vd, double *data, long *size);
  * batched over the population; other may have any bond dimensions. */
 int mqe_overlaps(mqe_engine *e, const mqe_mps *other, double *out);
 
-/* Device time of the last propagate kernel in milliseconds (HIP events on the
- * engine's stream), and the number of walkers it processed. */
+/* Device time of the walker kernel of the last mqe_propagate in milliseconds
+ * (HIP events on the engine's stream), and the population of the last
+ * propagation (mqe_propagate or a step of mqe_run). */
 int mqe_last_kernel_ms(mqe_engine *e, double *ms, int *walkers);
 
 /* Per-launch timing of the walker kernel inside mqe_step / mqe_run: enable
@@ -148,12 +159,13 @@ int mqe_last_kernel_ms(mqe_engine *e, double *ms, int *walkers);
 int mqe_set_timing(mqe_engine *e, int on);
 int mqe_kernel_time(mqe_engine *e, double *total_ms, int *launches);
 
-/* Snapshot of the walker population (tensors and environments as stored,
- * weights, overlaps, energy histories, local energies, the re-orthonormalisation
- * counter) and a caller step number, written to a file; mqe_load_state restores
- * it into an engine created with the same parameters and trial.  A restored
- * engine continues the walk exactly: the device random numbers are keyed by
- * seed, step and population position.  (The reference has no resume;
+/* Snapshot of the walker population (slots, tensors and environments as
+ * stored, weights, overlaps, energy histories, local energies, bond energies
+ * and recorded labels of the learned proposal, the propagation and device-draw
+ * counters) and a caller step number, written to a file; mqe_load_state
+ * restores it into an engine created with the same parameters and trial.  A
+ * restored engine continues the walk exactly: the device random numbers are
+ * keyed by seed, step and population position.  (The reference has no resume;
  * MPSQMC2.cpp:749-758 only writes the energy trace.) */
 int mqe_save_state(mqe_engine *e, const char *path, long step);
 int mqe_load_state(mqe_engine *e, const char *path, long *step);
@@ -245,6 +257,20 @@ int mqe_marsaglia_grid(int dim, int p, double *grid, int *n);
 /* MPSstate::LeftNormalize (MPSstate.cpp:216-229) on a flat MPS, in place;
  * *norm receives the final 1x1 R factor. */
 int mqe_left_normalize(int L, int d, const int *vd, double *data, double *norm);
+
+/* MPSstate::RightNormalize (MPSstate.cpp:231-246), in place: LQ per site from
+ * the right end, Lmat pushed into the left neighbour; *norm receives the final
+ * 1x1 factor. */
+int mqe_right_normalize(int L, int d, const int *vd, double *data, double *norm);
+
+/* MPSstate::ApplyMPO(HeisenbergMPO, Psi0) (MPSstate.cpp:413-471): H|psi> for
+ * H = sum_c J[c] S_first[c].S_second[c] - field sum_i S^z_i as an MPS of bond
+ * dims vd_out (bond k: channel w and state index r combined as w*vd[k] + r).
+ * The channels are those of a minimal MPO (2 + 3 x the sites whose couplings
+ * cross the bond), fewer than the reference's HeisenbergMPO; the vector is the
+ * same.  out may be NULL to query vd_out and *size (the flat length). d = 2. */
+int mqe_apply_mpo(int L, int d, int n_couplings, const int *first, const int *second, const double *J,
+                  double field, const int *vd, const double *data, int *vd_out, double *out, long *size);
 
 /* MPSstate::CompressState(Dt) (MPSstate.cpp:379-404): two SVD sweeps with the
  * 1e-13 absolute Schmidt cutoff.  vd_out receives the new bond dims and out

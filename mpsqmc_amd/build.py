@@ -14,6 +14,7 @@ LIB = os.path.join(HERE, "libmqe.so")
 
 SOURCES = ["mqe_engine.hip", "mqe_mlp.hip", "mqe_host.cpp", "mqe_dmrg.cpp", "mpsqmc.cpp"]
 WALK = os.path.join(HERE, "mpsqmc_walk")
+TOOL = os.path.join(HERE, "mpsqmc_tool")
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -43,6 +44,13 @@ def build(verbose=False):
         print(" ".join(cmd2), flush=True)
     subprocess.check_call(cmd2)
     os.replace(WALK + ".tmp", WALK)
+    # one MPSstate / Walker method per call, for the host-class tests
+    cmd3 = [hipcc, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), os.path.join(CSRC, "mpsqmc_tool.cpp"),
+            "-L", HERE, "-lmqe", "-Wl,-rpath,$ORIGIN", "-o", TOOL + ".tmp"]
+    if verbose:
+        print(" ".join(cmd3), flush=True)
+    subprocess.check_call(cmd3)
+    os.replace(TOOL + ".tmp", TOOL)
     return LIB
 
 

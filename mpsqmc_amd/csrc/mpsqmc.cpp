@@ -22,7 +22,7 @@ static void ck(int code, const char *what) {
 }
 
 // ---------------------------------------------------------------- Random
-Random::Random(unsigned int seed) : rng_(nullptr) { ck(mqe_rng_create(seed, &rng_), "Random"); }
+Random::Random(unsigned int seed) : rng_(nullptr), seed_(seed) { ck(mqe_rng_create(seed, &rng_), "Random"); }
 Random::~Random() { mqe_rng_destroy(rng_); }
 double Random::rand() {
     double u;
@@ -195,6 +195,104 @@ double MPSstate::InnerProduct(const MPSstate *o) const {
     return E[0];
 }
 
+double MPSstate::RightNormalize() {
+    double nrm = 0.0;
+    ck(mqe_right_normalize(L_, d_, vd_.data(), data_.data(), &nrm), "MPSstate::RightNormalize");
+    return nrm;
+}
+
+// MPSstate.cpp:286-305: |factor|^(1/L) on every site, the sign on site 0
+void MPSstate::ScalarMultiplication(double factor) {
+    const double f = std::pow(std::fabs(factor), 1.0 / L_);
+    const double sgn = factor > 0.0 ? 1.0 : (factor < 0.0 ? -1.0 : 0.0);
+    for (int k = 0; k < L_; ++k) {
+        const double a = (k == 0 ? sgn : 1.0) * f;
+        const long o = offset(k), n = (long)vd_[k] * vd_[k + 1] * d_;
+        for (long i = 0; i < n; ++i) data_[o + i] *= a;
+    }
+}
+
+void MPSstate::apply_site(int site, const double *op) {
+    const long blk = (long)vd_[site] * vd_[site + 1];
+    double *A = data_.data() + offset(site);
+    std::vector<double> out((size_t)blk * d_, 0.0);
+    for (int up = 0; up < d_; ++up)
+        for (int dn = 0; dn < d_; ++dn) {
+            const double v = op[up + d_ * dn];
+            if (v == 0.0) continue;
+            for (long e = 0; e < blk; ++e) out[(size_t)up * blk + e] += v * A[(size_t)dn * blk + e];
+        }
+    std::copy(out.begin(), out.end(), A);
+}
+
+void MPSstate::ApplyTwoSiteTrotterTerm(const TrotterHeisenberg *tr, int firstSite, int secondSite, int a, int b,
+                                       bool doHC) {
+    const double J = tr->gCoupling(firstSite, secondSite);
+    std::vector<double> opL((size_t)d_ * d_), opR((size_t)d_ * d_);
+    const double sa = std::sqrt(tr->gTwoSitePropSVD_Sing(J, a)), sb = std::sqrt(tr->gTwoSitePropSVD_Sing(J, b));
+    for (int up = 0; up < d_; ++up)
+        for (int dn = 0; dn < d_; ++dn) {
+            const int i = doHC ? dn : up, j = doHC ? up : dn;
+            opL[up + d_ * dn] = sa * tr->gTwoSitePropSVD_Left(J, a, i, j);
+            opR[up + d_ * dn] = sb * tr->gTwoSitePropSVD_Right(J, b, i, j);
+        }
+    apply_site(firstSite, opL.data());
+    apply_site(secondSite, opR.data());
+}
+
+void MPSstate::ApplyTwoSiteTrotterTerm(const TrotterHeisenberg *tr, int firstSite, int secondSite,
+                                       const GridGenerator *grid, int gridPoint) {
+    const double J = tr->gCoupling(firstSite, secondSite);
+    const int dim = grid->gDim();
+    std::vector<double> opL((size_t)d_ * d_, 0.0), opR((size_t)d_ * d_, 0.0);
+    for (int k = 0; k < dim; ++k) {
+        const double pf = std::sqrt(tr->gTwoSitePropSVD_Sing(J, k) * dim) * grid->gCoOfPoint(gridPoint, k);
+        if (pf == 0.0) continue;
+        for (int up = 0; up < d_; ++up)
+            for (int dn = 0; dn < d_; ++dn) {
+                opL[up + d_ * dn] += pf * tr->gTwoSitePropSVD_Left(J, k, up, dn);
+                opR[up + d_ * dn] += pf * tr->gTwoSitePropSVD_Right(J, k, up, dn);
+            }
+    }
+    apply_site(firstSite, opL.data());
+    apply_site(secondSite, opR.data());
+}
+
+void MPSstate::ApplyOneSiteTrotterTermEverywhere(const TrotterHeisenberg *tr) {
+    if (!tr->gIsMagneticField()) return;
+    std::vector<double> op((size_t)d_ * d_);
+    for (int up = 0; up < d_; ++up)
+        for (int dn = 0; dn < d_; ++dn) op[up + d_ * dn] = tr->gSingleSiteProp(up, dn);
+    for (int k = 0; k < L_; ++k) apply_site(k, op.data());
+}
+
+void MPSstate::ApplyMPO(const HeisenbergMPO *mpo, const MPSstate *Psi0) {
+    const int L = Psi0->L_;
+    std::vector<int> first, second;
+    std::vector<double> Jc;
+    for (int i = 0; i < L; ++i)
+        for (int j = i + 1; j < L; ++j)
+            if (mpo->gCoupling(i, j) != 0.0) {
+                first.push_back(i);
+                second.push_back(j);
+                Jc.push_back(mpo->gCoupling(i, j));
+            }
+    std::vector<int> vo(L + 1);
+    long n = 0;
+    ck(mqe_apply_mpo(L, Psi0->d_, (int)first.size(), first.data(), second.data(), Jc.data(), mpo->gField(),
+                     Psi0->vd_.data(), Psi0->data_.data(), vo.data(), nullptr, &n),
+       "MPSstate::ApplyMPO");
+    std::vector<double> out(n);
+    ck(mqe_apply_mpo(L, Psi0->d_, (int)first.size(), first.data(), second.data(), Jc.data(), mpo->gField(),
+                     Psi0->vd_.data(), Psi0->data_.data(), vo.data(), out.data(), &n),
+       "MPSstate::ApplyMPO");
+    L_ = L;
+    d_ = Psi0->d_;
+    vd_ = vo;
+    data_.swap(out);
+    Dtrunc_ = *std::max_element(vd_.begin(), vd_.end());
+}
+
 void MPSstate::ChangePhase() {  // MPSstate.cpp:307-315
     const long n = (long)vd_[0] * vd_[1] * d_;
     for (long i = 0; i < n; ++i) data_[i] = -data_[i];
@@ -226,6 +324,24 @@ void MPSstate::assign(const std::vector<int> &vd, const std::vector<double> &dat
     if ((int)vd.size() != L_ + 1) throw std::invalid_argument("MPSstate::assign: bond count");
     vd_ = vd;
     data_.assign(data.begin(), data.begin() + offset(L_));
+}
+
+// ---------------------------------------------------------------- Walker
+Walker::Walker(int length, int Dtrunc, int phys_d, Random *RN)
+    : state_(length, Dtrunc, phys_d, RN), overlap_(NAN), weight_(1.0), eh_(0.0) {}
+
+void Walker::setOverlap(const MPSstate *theTrial) {
+    state_.LeftNormalize();
+    double v = theTrial->InnerProduct(&state_);
+    if (v < 0.0) {  // importance sampling keeps O > 0
+        state_.ChangePhase();
+        v = -v;
+    }
+    overlap_ = v;
+}
+
+double Walker::calcIndividualProjectedEnergy(const MPSstate *HamTimesTrial) const {
+    return HamTimesTrial->InnerProduct(&state_) / overlap_;
 }
 
 // ---------------------------------------------------------------- DMRG
@@ -317,7 +433,7 @@ void MPSQMC2::setup(MPSstate *Psi0_in, int precision) {
     prm.max_bond = std::min(DW, DT_);
     prm.dtau = dtau;
     prm.field = theMPO->gField();
-    prm.seed = 20131015ull;
+    prm.seed = RN_ ? (unsigned long long)RN_->seed() : 20131015ull;  // device draws follow the caller's seed
     mqe_mps t;
     t.L = L;
     t.d = d;

@@ -464,3 +464,58 @@ extern "C" int mqe_dmrg(int L, int d, int n_couplings, const int *first, const i
     if (energy) *energy = E;
     return MQE_OK;
 }
+
+// MPSstate::ApplyMPO (MPSstate.cpp:413-471): H|psi> as an MPS whose bond k is
+// the MPO channel w and the state's bond index r combined as w * vd[k] + r (the
+// reference's index composition), with H = sum_c J_c S_i.S_j - field sum S^z_i
+// in the minimal-channel MPO above (2 + 3 x the sites whose couplings cross the
+// bond; the reference's HeisenbergMPO has more channels and gives the same
+// vector).  The boundary channels are "nothing placed" on the left and
+// "complete" on the right, so the outer bonds stay 1.
+extern "C" int mqe_apply_mpo(int L, int d, int n_couplings, const int *first, const int *second, const double *J,
+                             double field, const int *vd, const double *data, int *vd_out, double *out,
+                             long *size) {
+    if (L < 2 || d != 2 || n_couplings < 0 || !vd || !vd_out || (n_couplings > 0 && (!first || !second || !J))) {
+        mqe::set_error("mqe_apply_mpo: bad arguments (d must be 2)");
+        return MQE_EINVAL;
+    }
+    for (int c = 0; c < n_couplings; ++c)
+        if (first[c] < 0 || second[c] < 0 || first[c] >= L || second[c] >= L || first[c] == second[c]) {
+            mqe::set_error("mqe_apply_mpo: bad coupling");
+            return MQE_EINVAL;
+        }
+    if (vd[0] != 1 || vd[L] != 1) { mqe::set_error("mqe_apply_mpo: boundary bonds must be 1"); return MQE_EINVAL; }
+    const Mpo mpo = build_mpo(L, n_couplings, first, second, J, field);
+    std::vector<int> vo(L + 1);
+    for (int k = 0; k <= L; ++k) vo[k] = (k == 0 || k == L) ? 1 : mpo.W[k] * vd[k];
+    long total = 0;
+    for (int k = 0; k < L; ++k) total += (long)vo[k] * vo[k + 1] * d;
+    std::memcpy(vd_out, vo.data(), sizeof(int) * (L + 1));
+    if (size) *size = total;
+    if (!out) return MQE_OK;
+    std::memset(out, 0, sizeof(double) * total);
+    long oin = 0, oout = 0;
+    for (int k = 0; k < L; ++k) {
+        const int dl = vd[k], dr = vd[k + 1], nl = vo[k], nr = vo[k + 1];
+        const double *A = data + oin;
+        double *B = out + oout;
+        for (const Term &t : mpo.site[k]) {
+            // the outer bonds keep only the boundary channels
+            if (k == 0 && t.wl != 0) continue;
+            if (k == L - 1 && t.wr != 1) continue;
+            const int wl = k == 0 ? 0 : t.wl, wr = k == L - 1 ? 0 : t.wr;
+            for (int sp = 0; sp < 2; ++sp)
+                for (int s = 0; s < 2; ++s) {
+                    const double f = t.coef * OPS[t.op][sp + 2 * s];
+                    if (f == 0.0) continue;
+                    for (int c = 0; c < dr; ++c)
+                        for (int r = 0; r < dl; ++r)
+                            B[(wl * dl + r) + (size_t)nl * ((wr * dr + c) + (size_t)nr * sp)] +=
+                                f * A[r + (size_t)dl * (c + (size_t)dr * s)];
+                }
+        }
+        oin += (long)dl * dr * d;
+        oout += (long)nl * nr * d;
+    }
+    return MQE_OK;
+}

@@ -46,7 +46,27 @@ namespace mqe {
 static thread_local std::string g_err;
 void set_error(const char *msg) { g_err = msg ? msg : ""; }
 
-enum : int { F_PREP = 1, F_PROP = 2, F_CANON = 4 };
+enum : int { F_PREP = 1, F_PROP = 2 };
+
+// Step control block in device memory.  The launches of a step read their
+// step-dependent values from here instead of taking them as arguments, so the
+// same launches can be captured once in a graph and replayed (a one-thread
+// kernel advances the block at the start of every captured step).
+struct Ctl {
+    int step;         // device-RNG step word of the propagation; the energy's step number
+    int pstep;        // device-RNG step word of population control
+    long pc;          // propagations before this one (phase of the re-orthonormalisation)
+    int canon_every;  // walker slot s re-orthonormalises when (s + pc) % canon_every == canon_every - 1
+    int trace_cap;    // rows of the device trace
+};
+
+// Re-orthonormalisation is staggered over the walker slots: each propagation
+// re-orthonormalises the slots of one phase, so every step costs the same and a
+// walker that keeps its slot does it every canon_every steps (a copy moved to a
+// new slot waits at most 2 canon_every - 2 steps).
+__host__ __device__ inline bool canon_slot(const Ctl &c, int slot) {
+    return c.canon_every <= 1 || (int)((slot + c.pc) % c.canon_every) == c.canon_every - 1;
+}
 
 // Read-only description of the system (device pointers).
 struct Sys {
@@ -77,7 +97,8 @@ struct Pool {
     const long *woff, *eoff;
     const long *moff;      // general path: right norm environments M_k at Renv + moff[k] (w_k x w_k)
     unsigned long long seed;
-    int step;
+    const Ctl *ctl;        // step control block
+    const int *list;       // population positions in launch order (nullptr: block b = position b)
     int *status;
     char *scratch;
     long scratch_bytes;
@@ -594,10 +615,12 @@ __device__ void apply_field_everywhere(const Ctx<T> &C) {
 
 template <typename T>
 __global__ __launch_bounds__(MQE_NT, 2) void walker_kernel(Sys S, Pool P, const T *trial, int flags) {
-    const int b = blockIdx.x;
     const int n = *P.n;
-    if (b >= n) return;
+    if ((int)blockIdx.x >= n) return;
+    if (*P.status & 4) return;  // the population overflowed: keep the state as it was
+    const int b = P.list ? P.list[blockIdx.x] : (int)blockIdx.x;  // population position
     const int slot = P.order[b];
+    const Ctl ctl = *P.ctl;
     const int L = S.L;
 
     __shared__ int tv[MQE_MAXL + 1], wv[MQE_MAXL + 1];
@@ -702,7 +725,7 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker_kernel(Sys S, Pool P, const 
         for (int g = threadIdx.x; g < Ng; g += MQE_NT) pdf[g] *= inv;
         __syncthreads();
         block_inclusive_scan(Ng, pdf, cum, red);
-        const double u = P.u ? P.u[(size_t)b * S.nC + c] : philox_uniform(P.seed, (uint32_t)b, (uint32_t)c, (uint32_t)P.step, 0u);
+        const double u = P.u ? P.u[(size_t)b * S.nC + c] : philox_uniform(P.seed, (uint32_t)b, (uint32_t)c, (uint32_t)ctl.step, 0u);
         double cnt = 0.0;
         for (int g = threadIdx.x; g < Ng - 1; g += MQE_NT) cnt += (cum[g] <= u) ? 1.0 : 0.0;
         cnt = block_sum(cnt, red);
@@ -795,7 +818,7 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker_kernel(Sys S, Pool P, const 
     }
     // ---------------- local energy (Walker.cpp:72-77) ----------------
     double On, Ou;
-    const double e = energy_sweep(C, &On, &Ou, (flags & F_CANON) != 0);
+    const double e = energy_sweep(C, &On, &Ou, canon_slot(ctl, slot));
     if (threadIdx.x == 0) {
         P.w[slot] = w;
         P.O[slot] = On;
@@ -892,10 +915,55 @@ struct Book {
     double *sums;     // [0] sum w  [1] scaling  [2] E_T  [3] E_P  [4] Omega  [5] N
     double *trace;    // 6 doubles per step
     int *ncopy, *newpos, *used, *freelist, *copy_src, *copy_dst, *n_extra, *n_new, *n_used;
+    double *nw;       // population control: new weights by position, committed once the copies fit
     int *status;
+    const Ctl *ctl;
     int cap, n_desired;
     double dtau;
 };
+
+__global__ void set_ctl_kernel(Ctl *ctl, Ctl v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *ctl = v;
+}
+
+// start of a captured step: next step number and propagation
+__global__ void advance_ctl_kernel(Ctl *ctl) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        ctl->step += 1;
+        ctl->pstep += 1;
+        ctl->pc += 1;
+    }
+}
+
+// Population positions in launch order: the walkers that re-orthonormalise in
+// this propagation (several times the work of the others) first, so that they
+// start in the first round of resident workgroups instead of extending the
+// last one; each group keeps population order.
+__global__ __launch_bounds__(RED_NT) void launch_list_kernel(const int *n_ptr, const int *order, const Ctl *ctl,
+                                                             int *list) {
+    __shared__ int shi[RED_NT / 64];
+    __shared__ int carry_h, carry_l;
+    const int n = *n_ptr;
+    const Ctl c = *ctl;
+    int nh = 0;
+    for (int b = threadIdx.x; b < n; b += RED_NT) nh += canon_slot(c, order[b]) ? 1 : 0;
+    int ex;
+    const int nheavy = rscan_excl(nh, &ex, shi);
+    if (threadIdx.x == 0) { carry_h = 0; carry_l = 0; }
+    __syncthreads();
+    for (int base = 0; base < n; base += RED_NT) {
+        const int b = base + threadIdx.x;
+        const int h = (b < n && canon_slot(c, order[b])) ? 1 : 0;
+        const int toth = rscan_excl(h, &ex, shi);
+        if (b < n) list[h ? carry_h + ex : nheavy + carry_l + (threadIdx.x - ex)] = b;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            carry_h += toth;
+            carry_l += min(RED_NT, n - base) - toth;
+        }
+        __syncthreads();
+    }
+}
 
 // sum of weights; if device_scaling also scaling and E_T (MPSQMC2.cpp:480-484)
 __global__ __launch_bounds__(RED_NT) void sumw_kernel(Book B) {
@@ -914,10 +982,15 @@ __global__ __launch_bounds__(RED_NT) void sumw_kernel(Book B) {
     }
 }
 
-// EnergyFunctionAndHistory (MPSQMC2.cpp:688-747)
-__global__ __launch_bounds__(RED_NT) void energy_kernel(Book B, int step, int dofluct, int trace_row) {
+// EnergyFunctionAndHistory (MPSQMC2.cpp:688-747).  from_ctl: step number from
+// the control block and a trace row for it (a step of mqe_step / mqe_run).
+__global__ __launch_bounds__(RED_NT) void energy_kernel(Book B, int step, int dofluct, int trace_row, int from_ctl) {
     __shared__ double sh[RED_NT / 64];
     const int n = *B.n;
+    if (from_ctl) {
+        step = B.ctl->step;
+        trace_row = (step - 1) % B.ctl->trace_cap;
+    }
     double num = 0.0, den = 0.0, s1 = 0.0, s2 = 0.0;
     for (int b = threadIdx.x; b < n; b += RED_NT) {
         const int sl = B.order[b];
@@ -949,10 +1022,15 @@ __global__ __launch_bounds__(RED_NT) void energy_kernel(Book B, int step, int do
 // SeparatePopulationControl (MPSQMC2.cpp:613-686): copy counts, new order,
 // slot assignment for the extra copies.  scaling from host (>0) or B.sums[1].
 __global__ __launch_bounds__(RED_NT) void popcontrol_kernel(Book B, double host_scaling, const double *u, int nu,
-                                                            unsigned long long seed, int step) {
+                                                            unsigned long long seed) {
     __shared__ int shi[RED_NT / 64];
     __shared__ int carry_draw, carry_pos, carry_extra, carry_free;
     const int n = *B.n;
+    if (*B.status & 4) {  // an earlier overflow: leave the population as it is
+        if (threadIdx.x == 0) { *B.n_extra = 0; *B.n_new = B.cap + 1; *B.n_used = 0; }  // order and n unchanged
+        return;
+    }
+    const int step = B.ctl->pstep;
     const double scaling = host_scaling > 0.0 ? host_scaling : B.sums[1];
     if (threadIdx.x == 0) { carry_draw = 0; carry_pos = 0; carry_extra = 0; carry_free = 0; }
     for (int s = threadIdx.x; s < B.cap; s += RED_NT) B.used[s] = 0;
@@ -979,7 +1057,7 @@ __global__ __launch_bounds__(RED_NT) void popcontrol_kernel(Book B, double host_
                 nw = (sw < 0.25) ? 1.0 : sw / c;
             }
             B.ncopy[b] = c;
-            B.w[B.order[b]] = nw;  // applied to survivors; copies read it before the copy kernel
+            B.nw[b] = nw;  // committed below once the copies are known to fit
         }
         int pos;
         const int totc = rscan_excl(b < n ? c : 0, &pos, shi);
@@ -994,6 +1072,8 @@ __global__ __launch_bounds__(RED_NT) void popcontrol_kernel(Book B, double host_
         if (threadIdx.x == 0) { atomicOr(B.status, 4); *B.n_extra = 0; }
         return;
     }
+    // new weights (survivors; the copies read them in the copy kernel)
+    for (int b = threadIdx.x; b < n; b += RED_NT) B.w[B.order[b]] = B.nw[b];
     // survivors keep their slot
     for (int b = threadIdx.x; b < n; b += RED_NT)
         if (B.ncopy[b] > 0) B.used[B.order[b]] = 1;
@@ -1190,15 +1270,20 @@ struct mqe_engine {
     int trace_cap = 0;
     int *d_ncopy = nullptr, *d_newpos = nullptr, *d_used = nullptr, *d_free = nullptr, *d_csrc = nullptr,
         *d_cdst = nullptr, *d_nextra = nullptr, *d_nnew = nullptr, *d_nused = nullptr;
-    double *d_tmpout = nullptr;
+    double *d_tmpout = nullptr, *d_nw = nullptr;
+    Ctl *d_ctl = nullptr;                      // step control block (device)
+    Ctl ctl_host{};                            // its last value set from the host
+    int *d_list = nullptr;                     // launch order of the walker kernel
+    long prop_draws = 0, pop_draws = 0;        // step words of mqe_propagate / mqe_population_control without host draws
     int n_host = 0;
     bool loaded = false, prepared = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_ms = 0.f;
     int last_walkers = 0;
-    hipGraphExec_t graph_exec = nullptr;
+    hipGraphExec_t graph_exec = nullptr;       // mqe_run with use_graph: graph_steps captured steps
+    int graph_steps = 0;
     // per-launch timing of the walker kernel inside mqe_step / mqe_run
-    int canon_every = 64;  // tile path: FP64 LQ re-canonicalisation every this many steps
+    int canon_every = 64;  // FP64 LQ re-orthonormalisation of every walker every this many steps (staggered)
     long prop_count = 0;
     bool timing = false;
     std::vector<hipEvent_t> tev;
@@ -1206,7 +1291,7 @@ struct mqe_engine {
     Sys sys;
 };
 
-static Pool make_pool(mqe_engine *e, const double *u, int step) {
+static Pool make_pool(mqe_engine *e, const double *u) {
     Pool P;
     P.phi = e->d_phi;
     P.Renv = e->d_Renv;
@@ -1226,7 +1311,8 @@ static Pool make_pool(mqe_engine *e, const double *u, int step) {
     P.eoff = e->d_eoff;
     P.moff = e->d_moff;
     P.seed = e->p.seed;
-    P.step = step;
+    P.ctl = e->d_ctl;
+    P.list = nullptr;
     P.status = e->d_status;
     P.scratch = e->d_scratch;
     P.scratch_bytes = e->scratch_bytes;
@@ -1264,7 +1350,9 @@ static Book make_book(mqe_engine *e) {
     B.n_extra = e->d_nextra;
     B.n_new = e->d_nnew;
     B.n_used = e->d_nused;
+    B.nw = e->d_nw;
     B.status = e->d_status;
+    B.ctl = e->d_ctl;
     B.cap = e->p.capacity;
     B.n_desired = e->p.n_desired;
     B.dtau = e->p.dtau;
@@ -1272,20 +1360,35 @@ static Book make_book(mqe_engine *e) {
 }
 
 template <typename T>
-static void launch_walker(mqe_engine *e, const double *u, int step, int flags) {
-    Pool P = make_pool(e, u, step);
+static void launch_walker(mqe_engine *e, const Pool &P, int flags) {
     hipLaunchKernelGGL(walker_kernel<T>, dim3(e->p.capacity), dim3(MQE_NT), e->gen_lds_bytes, e->stream, e->sys, P,
                        (const T *)e->d_trial, flags);
 }
 
-static int launch_walker_any(mqe_engine *e, const double *u, int step, int flags) {
-    if (flags & F_PROP) {
-        if (e->canon_every <= 1 || (e->prop_count % e->canon_every) == e->canon_every - 1) flags |= F_CANON;
+// Copy a control block value to the device with a kernel (no memset or copy
+// node, so the same call may sit in front of a graph replay).
+static int push_ctl(mqe_engine *e) {
+    e->ctl_host.canon_every = e->canon_every;
+    e->ctl_host.trace_cap = e->trace_cap;
+    hipLaunchKernelGGL(set_ctl_kernel, dim3(1), dim3(64), 0, e->stream, e->d_ctl, e->ctl_host);
+    HIPCHK(hipGetLastError());
+    return MQE_OK;
+}
+
+// F_PREP: canonical form and local energies of the loaded walkers.  F_PROP: one
+// propagation with the step words and phase of the control block, walkers in
+// the launch order of launch_list_kernel.
+static int launch_walker_any(mqe_engine *e, const double *u, int flags) {
+    Pool P = make_pool(e, u);
+    const bool prop = (flags & F_PROP) != 0;
+    if (prop) {
         ++e->prop_count;
+        hipLaunchKernelGGL(launch_list_kernel, dim3(1), dim3(RED_NT), 0, e->stream, e->d_n, e->d_order, e->d_ctl,
+                           e->d_list);
+        HIPCHK(hipGetLastError());
+        P.list = e->d_list;
     }
     if (e->tile32) {
-        Pool P = make_pool(e, u, step);
-        const bool prop = (flags & F_PROP) != 0;
         const bool collect = prop && e->max_samples > 0, learn = prop && e->learn;
         if (collect || (learn && e->net)) {
             hipLaunchKernelGGL(feature_kernel, dim3(std::min(e->p.capacity, 4096)), dim3(64), 0, e->stream, P, e->nC,
@@ -1309,8 +1412,8 @@ static int launch_walker_any(mqe_engine *e, const double *u, int step, int flags
             hipLaunchKernelGGL(advance_samples_kernel, dim3(1), dim3(64), 0, e->stream, e->d_n, e->nC, e->d_nsamp,
                                e->max_samples);
         }
-    } else if (e->fp32) launch_walker<float>(e, u, step, flags);
-    else launch_walker<double>(e, u, step, flags);
+    } else if (e->fp32) launch_walker<float>(e, P, flags);
+    else launch_walker<double>(e, P, flags);
     HIPCHK(hipGetLastError());
     return MQE_OK;
 }
@@ -1342,13 +1445,14 @@ static int upload_mps_typed(const double *src, long n, void **dst) {
 template <typename T>
 static void launch_copy(mqe_engine *e) {
     Book B = make_book(e);
-    Pool P = make_pool(e, nullptr, 0);
+    Pool P = make_pool(e, nullptr);
     hipLaunchKernelGGL(copy_walkers_kernel<T>, dim3(2048), dim3(MQE_NT), 0, e->stream, B, P, e->L);
 }
 
-static int enqueue_popcontrol(mqe_engine *e, double scaling, const double *du, int nu, int step) {
+// population control with the step word of the control block
+static int enqueue_popcontrol(mqe_engine *e, double scaling, const double *du, int nu) {
     Book B = make_book(e);
-    hipLaunchKernelGGL(popcontrol_kernel, dim3(1), dim3(RED_NT), 0, e->stream, B, scaling, du, nu, e->p.seed, step);
+    hipLaunchKernelGGL(popcontrol_kernel, dim3(1), dim3(RED_NT), 0, e->stream, B, scaling, du, nu, e->p.seed);
     HIPCHK(hipGetLastError());
     if (e->fp32) launch_copy<float>(e);
     else launch_copy<double>(e);
@@ -1566,6 +1670,9 @@ int mqe_create(const mqe_params *p, const mqe_mps *trial, int n_couplings, const
     CK(hipMalloc((void **)&e->d_nnew, sizeof(int)));
     CK(hipMalloc((void **)&e->d_nused, sizeof(int)));
     CK(hipMalloc((void **)&e->d_tmpout, sizeof(double) * cap));
+    CK(hipMalloc((void **)&e->d_nw, sizeof(double) * cap));
+    CK(hipMalloc((void **)&e->d_list, sizeof(int) * cap));
+    CK(hipMalloc((void **)&e->d_ctl, sizeof(Ctl)));
 
     Sys &S = e->sys;
     S.L = L;
@@ -1600,6 +1707,8 @@ int mqe_create(const mqe_params *p, const mqe_mps *trial, int n_couplings, const
     S.h = p->field;
     S.ssp[0] = std::exp(0.5 * p->field * p->dtau * (-0.5));  // TrotterHeisenberg.cpp:86
     S.ssp[1] = std::exp(0.5 * p->field * p->dtau * (0.5));
+    if (push_ctl(e) != MQE_OK) return fail(MQE_EDEVICE);
+    CK(hipStreamSynchronize(e->stream));
     *out = e;
 #undef CK
     return MQE_OK;
@@ -1614,7 +1723,8 @@ void mqe_destroy(mqe_engine *e) {
                     e->d_Renv, e->d_Lenv, e->d_wvd, e->d_w, e->d_O, e->d_eh, e->d_EL, e->d_order, e->d_order2,
                     e->d_n, e->d_idx, e->d_u, e->d_scratch, e->d_status, e->d_sums, e->d_trace, e->d_ncopy,
                     e->d_newpos, e->d_used, e->d_free, e->d_csrc, e->d_cdst, e->d_nextra, e->d_nnew, e->d_nused,
-                    e->d_tmpout, e->d_bond, e->d_feat, e->d_pred, e->d_sX, e->d_sY, e->d_nsamp, e->d_plab};
+                    e->d_tmpout, e->d_bond, e->d_feat, e->d_pred, e->d_sX, e->d_sY, e->d_nsamp, e->d_plab,
+                    e->d_nw, e->d_list, e->d_ctl};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     for (hipEvent_t ev : e->tev) (void)hipEventDestroy(ev);
@@ -1623,6 +1733,27 @@ void mqe_destroy(mqe_engine *e) {
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
 }
+
+}  // extern "C"
+
+// Tile path: the boundary environments L_0 = 1, R_L = 1 and M_L = 1 of every
+// slot (the kernel reads them from global memory and never writes them; walker
+// copies carry the right environments but not the left ones), the rest zero.
+static int init_tile_env(mqe_engine *e) {
+    if (!e->tile32) return MQE_OK;
+    const int L = e->L;
+    std::vector<float> env((size_t)e->eflat * e->p.capacity, 0.f);
+    for (int sl = 0; sl < e->p.capacity; ++sl) {
+        env[(size_t)sl * e->eflat] = 1.f;
+        env[(size_t)sl * e->eflat + (size_t)L * 1024] = 1.f;
+        env[(size_t)sl * e->eflat + (size_t)(2 * L + 1) * 1024] = 1.f;  // M_L = 1
+    }
+    HIPCHK(hipMemcpy(e->d_Lenv, env.data(), sizeof(float) * env.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->d_Renv, env.data(), sizeof(float) * env.size(), hipMemcpyHostToDevice));
+    return MQE_OK;
+}
+
+extern "C" {
 
 int mqe_load_walkers(mqe_engine *e, int n, const int *vd, const double *data, const double *w, const double *O,
                      const double *eh) {
@@ -1662,17 +1793,7 @@ int mqe_load_walkers(mqe_engine *e, int n, const int *vd, const double *data, co
             off += sz;
         }
     }
-    if (e->tile32) {
-        // L_0 = R_L = 1 in every slot (boundary environments read from global memory)
-        std::vector<float> env((size_t)e->eflat * e->p.capacity, 0.f);
-        for (int sl = 0; sl < e->p.capacity; ++sl) {
-            env[(size_t)sl * e->eflat] = 1.f;
-            env[(size_t)sl * e->eflat + (size_t)L * 1024] = 1.f;
-            env[(size_t)sl * e->eflat + (size_t)(2 * L + 1) * 1024] = 1.f;  // M_L = 1
-        }
-        HIPCHK(hipMemcpy(e->d_Lenv, env.data(), sizeof(float) * env.size(), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(e->d_Renv, env.data(), sizeof(float) * env.size(), hipMemcpyHostToDevice));
-    }
+    ECHK(init_tile_env(e));
     HIPCHK(hipMemcpy(e->d_phi, buf.data(), buf.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(e->d_wvd, vds.data(), sizeof(int) * vds.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(e->d_w, w, sizeof(double) * n, hipMemcpyHostToDevice));
@@ -1700,9 +1821,9 @@ static int ensure_u(mqe_engine *e, long need) {
 
 int mqe_prepare(mqe_engine *e, double *E_start) {
     if (!e || !e->loaded) { set_error("mqe_prepare: no walkers loaded"); return MQE_ESTATE; }
-    ECHK(launch_walker_any(e, nullptr, 0, F_PREP));
+    ECHK(launch_walker_any(e, nullptr, F_PREP));
     Book B = make_book(e);
-    hipLaunchKernelGGL(energy_kernel, dim3(1), dim3(RED_NT), 0, e->stream, B, 1, 0, -1);
+    hipLaunchKernelGGL(energy_kernel, dim3(1), dim3(RED_NT), 0, e->stream, B, 1, 0, -1, 0);
     HIPCHK(hipGetLastError());
     double sums[8];
     HIPCHK(hipMemcpyAsync(sums, e->d_sums, sizeof(sums), hipMemcpyDeviceToHost, e->stream));
@@ -1724,9 +1845,14 @@ int mqe_propagate(mqe_engine *e, const double *uniforms, double *sum_w) {
         HIPCHK(hipMemcpyAsync(e->d_u, uniforms, sizeof(double) * (size_t)e->n_host * e->nC, hipMemcpyHostToDevice,
                               e->stream));
         du = e->d_u;
+    } else {
+        // device draws: the k-th such call uses step word k, as step k of mqe_run does
+        e->ctl_host.step = (int)++e->prop_draws;
     }
+    e->ctl_host.pc = e->prop_count;
+    ECHK(push_ctl(e));
     HIPCHK(hipEventRecord(e->ev0, e->stream));
-    ECHK(launch_walker_any(e, du, 0, F_PROP));
+    ECHK(launch_walker_any(e, du, F_PROP));
     HIPCHK(hipEventRecord(e->ev1, e->stream));
     Book B = make_book(e);
     hipLaunchKernelGGL(sumw_kernel, dim3(1), dim3(RED_NT), 0, e->stream, B);
@@ -1735,7 +1861,7 @@ int mqe_propagate(mqe_engine *e, const double *uniforms, double *sum_w) {
     HIPCHK(hipMemcpyAsync(sums, e->d_sums, sizeof(sums), hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     HIPCHK(hipEventElapsedTime(&e->last_ms, e->ev0, e->ev1));
-    e->last_walkers = e->n_host;
+    e->last_walkers = (int)sums[5];
     if (sum_w) *sum_w = sums[0];
     return check_status(e);
 }
@@ -1744,7 +1870,7 @@ int mqe_energy(mqe_engine *e, int step, int do_fluct, double *E_P, double *omega
     if (!e || !e->prepared) { set_error("mqe_energy: call mqe_prepare first"); return MQE_ESTATE; }
     if (step < 1) { set_error("mqe_energy: step must be >= 1"); return MQE_EINVAL; }
     Book B = make_book(e);
-    hipLaunchKernelGGL(energy_kernel, dim3(1), dim3(RED_NT), 0, e->stream, B, step, do_fluct ? 1 : 0, -1);
+    hipLaunchKernelGGL(energy_kernel, dim3(1), dim3(RED_NT), 0, e->stream, B, step, do_fluct ? 1 : 0, -1, 0);
     HIPCHK(hipGetLastError());
     double sums[8];
     HIPCHK(hipMemcpyAsync(sums, e->d_sums, sizeof(sums), hipMemcpyDeviceToHost, e->stream));
@@ -1763,16 +1889,19 @@ int mqe_population_control(mqe_engine *e, double scaling, const double *uniforms
         ECHK(ensure_u(e, n_uniforms));
         HIPCHK(hipMemcpyAsync(e->d_u, uniforms, sizeof(double) * n_uniforms, hipMemcpyHostToDevice, e->stream));
         du = e->d_u;
+    } else if (!uniforms) {
+        // device draws: the k-th such call uses step word k, as step k of mqe_run does
+        e->ctl_host.pstep = (int)++e->pop_draws;
+        ECHK(push_ctl(e));
     }
-    ECHK(enqueue_popcontrol(e, scaling, du, uniforms ? n_uniforms : 0, 0));
+    ECHK(enqueue_popcontrol(e, scaling, du, uniforms ? n_uniforms : 0));
     int nn = 0, nu = 0;
     HIPCHK(hipMemcpyAsync(&nn, e->d_nnew, sizeof(int), hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipMemcpyAsync(&nu, e->d_nused, sizeof(int), hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     if (n_used) *n_used = nu;
     if (nn > e->p.capacity) {
-        HIPCHK(hipMemset(e->d_status, 0, sizeof(int)));
-        set_error("population exceeds walker capacity");
+        set_error("population exceeds walker capacity (the population is left as it was before this call)");
         return MQE_ECAPACITY;
     }
     e->n_host = nn;
@@ -1780,8 +1909,21 @@ int mqe_population_control(mqe_engine *e, double scaling, const double *uniforms
     return MQE_OK;
 }
 
-static int enqueue_step(mqe_engine *e, int step) {
-    if (e->timing) {
+// One Walk step.  Eager: the control block is set to this step; in a graph
+// capture (captured = true) a kernel advances it instead, so the captured
+// launches carry no step-dependent argument.
+static int enqueue_step(mqe_engine *e, int step, bool captured) {
+    if (captured) {
+        hipLaunchKernelGGL(advance_ctl_kernel, dim3(1), dim3(64), 0, e->stream, e->d_ctl);
+        HIPCHK(hipGetLastError());
+    } else {
+        e->ctl_host.step = e->ctl_host.pstep = step;
+        e->ctl_host.pc = e->prop_count;
+        e->prop_draws = e->pop_draws = step;
+        ECHK(push_ctl(e));
+    }
+    const bool timed = e->timing && !captured;
+    if (timed) {
         if (e->tev_used + 2 > (int)e->tev.size()) {
             for (int q = 0; q < 64; ++q) {
                 hipEvent_t ev;
@@ -1791,30 +1933,73 @@ static int enqueue_step(mqe_engine *e, int step) {
         }
         HIPCHK(hipEventRecord(e->tev[e->tev_used], e->stream));
     }
-    ECHK(launch_walker_any(e, nullptr, step, F_PROP));
-    if (e->timing) {
+    ECHK(launch_walker_any(e, nullptr, F_PROP));
+    if (timed) {
         HIPCHK(hipEventRecord(e->tev[e->tev_used + 1], e->stream));
         e->tev_used += 2;
     }
     Book B = make_book(e);
     hipLaunchKernelGGL(sumw_kernel, dim3(1), dim3(RED_NT), 0, e->stream, B);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(energy_kernel, dim3(1), dim3(RED_NT), 0, e->stream, B, step, 1, (step - 1) % e->trace_cap);
+    hipLaunchKernelGGL(energy_kernel, dim3(1), dim3(RED_NT), 0, e->stream, B, step, 1, -1, 1);
     HIPCHK(hipGetLastError());
-    ECHK(enqueue_popcontrol(e, -1.0, nullptr, 0, step));
+    ECHK(enqueue_popcontrol(e, -1.0, nullptr, 0));
     return MQE_OK;
 }
 
 int mqe_step(mqe_engine *e, int step) {
     if (!e || !e->prepared) { set_error("mqe_step: call mqe_prepare first"); return MQE_ESTATE; }
     if (step < 1) { set_error("mqe_step: step must be >= 1"); return MQE_EINVAL; }
-    return enqueue_step(e, step);
+    return enqueue_step(e, step, false);
+}
+
+// Capture n_steps steps once (kept until a run with another count) and replay
+// them: the control block is set to the step before first_step, and each
+// captured step advances it.
+static int run_graph(mqe_engine *e, int first_step, int n_steps) {
+    if (e->learn || e->max_samples > 0) {
+        set_error("mqe_run: graph replay runs the reference proposal without sample collection");
+        return MQE_EINVAL;
+    }
+    if (!e->graph_exec || e->graph_steps != n_steps) {
+        if (e->graph_exec) HIPCHK(hipGraphExecDestroy(e->graph_exec));
+        e->graph_exec = nullptr;
+        e->graph_steps = 0;
+        HIPCHK(hipStreamSynchronize(e->stream));
+        const long pc0 = e->prop_count;
+        HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        int r = MQE_OK;
+        for (int s = 0; s < n_steps && r == MQE_OK; ++s) r = enqueue_step(e, 0, true);
+        hipGraph_t g = nullptr;
+        const hipError_t ce = hipStreamEndCapture(e->stream, &g);
+        e->prop_count = pc0;
+        if (r != MQE_OK || ce != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            if (r == MQE_OK) HIPCHK(ce);
+            return r;
+        }
+        const hipError_t ie = hipGraphInstantiate(&e->graph_exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHK(ie);
+        e->graph_steps = n_steps;
+    }
+    e->ctl_host.step = e->ctl_host.pstep = first_step - 1;
+    e->ctl_host.pc = e->prop_count - 1;
+    ECHK(push_ctl(e));
+    HIPCHK(hipGraphLaunch(e->graph_exec, e->stream));
+    e->prop_count += n_steps;
+    e->prop_draws = e->pop_draws = first_step + n_steps - 1;
+    e->ctl_host.step = e->ctl_host.pstep = first_step + n_steps - 1;
+    e->ctl_host.pc = e->prop_count - 1;
+    return MQE_OK;
 }
 
 int mqe_run(mqe_engine *e, int first_step, int n_steps, int use_graph) {
     if (!e || !e->prepared) { set_error("mqe_run: call mqe_prepare first"); return MQE_ESTATE; }
-    (void)use_graph;
-    for (int s = 0; s < n_steps; ++s) ECHK(enqueue_step(e, first_step + s));
+    if (first_step < 1 || n_steps < 0) { set_error("mqe_run: bad step range"); return MQE_EINVAL; }
+    if (n_steps == 0) return MQE_OK;
+    if (use_graph) return run_graph(e, first_step, n_steps);
+    for (int s = 0; s < n_steps; ++s) ECHK(enqueue_step(e, first_step + s, false));
     return MQE_OK;
 }
 
@@ -1860,9 +2045,25 @@ int mqe_num_couplings(mqe_engine *e, int *n) {
     return MQE_OK;
 }
 
+// the population size after the work queued so far
+static int refresh_n(mqe_engine *e) {
+    HIPCHK(hipStreamSynchronize(e->stream));
+    HIPCHK(hipMemcpy(&e->n_host, e->d_n, sizeof(int), hipMemcpyDeviceToHost));
+    return MQE_OK;
+}
+
+// the population of the last propagation (recorded by its weight sum)
+static int refresh_last_walkers(mqe_engine *e) {
+    double n = 0.0;
+    HIPCHK(hipStreamSynchronize(e->stream));
+    HIPCHK(hipMemcpy(&n, e->d_sums + 5, sizeof(double), hipMemcpyDeviceToHost));
+    e->last_walkers = (int)n;
+    return MQE_OK;
+}
+
 int mqe_read_scalars(mqe_engine *e, double *w, double *O, double *eh, double *EL) {
     if (!e) { set_error("mqe_read_scalars: null engine"); return MQE_EINVAL; }
-    HIPCHK(hipStreamSynchronize(e->stream));
+    ECHK(refresh_n(e));
     const int n = e->n_host, cap = e->p.capacity;
     std::vector<int> ord(n);
     HIPCHK(hipMemcpy(ord.data(), e->d_order, sizeof(int) * n, hipMemcpyDeviceToHost));
@@ -1879,14 +2080,15 @@ int mqe_read_scalars(mqe_engine *e, double *w, double *O, double *eh, double *EL
 
 int mqe_read_indices(mqe_engine *e, int *idx) {
     if (!e || !idx) { set_error("mqe_read_indices: bad arguments"); return MQE_EINVAL; }
-    HIPCHK(hipStreamSynchronize(e->stream));
+    ECHK(refresh_last_walkers(e));
     HIPCHK(hipMemcpy(idx, e->d_idx, sizeof(int) * (size_t)e->last_walkers * e->nC, hipMemcpyDeviceToHost));
     return MQE_OK;
 }
 
 int mqe_read_walker(mqe_engine *e, int i, int *vd, double *data, long *size) {
-    if (!e || i < 0 || i >= e->n_host || !vd) { set_error("mqe_read_walker: bad arguments"); return MQE_EINVAL; }
-    HIPCHK(hipStreamSynchronize(e->stream));
+    if (!e || !vd) { set_error("mqe_read_walker: bad arguments"); return MQE_EINVAL; }
+    ECHK(refresh_n(e));
+    if (i < 0 || i >= e->n_host) { set_error("mqe_read_walker: walker index out of range"); return MQE_EINVAL; }
     const int L = e->L;
     int slot = 0;
     HIPCHK(hipMemcpy(&slot, e->d_order + i, sizeof(int), hipMemcpyDeviceToHost));
@@ -1921,6 +2123,7 @@ int mqe_overlaps(mqe_engine *e, const mqe_mps *other, double *out) {
     if (!e || !other || !out || other->L != e->L || other->d != 2) { set_error("mqe_overlaps: bad arguments"); return MQE_EINVAL; }
     const int L = e->L;
     if (other->vd[0] != 1 || other->vd[L] != 1) { set_error("mqe_overlaps: boundary bonds must be 1"); return MQE_EINVAL; }
+    ECHK(refresh_n(e));
     int Dmax = e->Dw;
     std::vector<long> off(L);
     long tot = 0;
@@ -1943,7 +2146,7 @@ int mqe_overlaps(mqe_engine *e, const mqe_mps *other, double *out) {
     HIPCHK(hipMalloc((void **)&d_ooff, sizeof(long) * L));
     HIPCHK(hipMemcpy(d_ovd, other->vd, sizeof(int) * (L + 1), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(d_ooff, off.data(), sizeof(long) * L, hipMemcpyHostToDevice));
-    Pool P = make_pool(e, nullptr, 0);
+    Pool P = make_pool(e, nullptr);
     P.scratch = scr;
     P.scratch_bytes = (long)need;
     if (e->fp32)
@@ -1986,6 +2189,7 @@ int mqe_kernel_time(mqe_engine *e, double *total_ms, int *launches) {
 
 int mqe_last_kernel_ms(mqe_engine *e, double *ms, int *walkers) {
     if (!e) { set_error("mqe_last_kernel_ms: null engine"); return MQE_EINVAL; }
+    ECHK(refresh_last_walkers(e));
     if (ms) *ms = e->last_ms;
     if (walkers) *walkers = e->last_walkers;
     return MQE_OK;
@@ -1993,14 +2197,19 @@ int mqe_last_kernel_ms(mqe_engine *e, double *ms, int *walkers) {
 
 // ---- population snapshot ------------------------------------------------------
 // File: header (magic, sizes, counters), then per walker in population order its
-// tensors, right and left environments (as stored, in the engine's gauge), bond
-// dims and the four scalars.  Walkers are copied one at a time.
+// slot, tensors, right and left environments (as stored, in the engine's gauge),
+// bond dims, the four scalars and, on the tile path, its bond energies and
+// recorded labels.  Walkers go back to their slots, so the free slots, the
+// staggered re-orthonormalisation and the device random numbers continue exactly.
 struct StateHeader {
     char magic[8];
     int L, d, precision, tile32;
     long wflat, eflat;
     int n, prepared;
     long prop_count, step;
+    int capacity, nC;
+    long prop_draws, pop_draws;
+    int canon_every, pad;
 };
 
 // ---- learned proposal -------------------------------------------------------
@@ -2134,17 +2343,16 @@ int mqe_read_bond_energies(mqe_engine *e, double *out) {
 int mqe_save_state(mqe_engine *e, const char *path, long step) {
     if (!e || !path) { set_error("mqe_save_state: bad arguments"); return MQE_EINVAL; }
     if (!e->loaded) { set_error("mqe_save_state: no walkers loaded"); return MQE_ESTATE; }
-    HIPCHK(hipStreamSynchronize(e->stream));
-    int n = 0;
-    HIPCHK(hipMemcpy(&n, e->d_n, sizeof(int), hipMemcpyDeviceToHost));
+    ECHK(refresh_n(e));
+    const int n = e->n_host, L = e->L, nC = e->nC;
     std::vector<int> order(std::max(n, 1));
     HIPCHK(hipMemcpy(order.data(), e->d_order, sizeof(int) * n, hipMemcpyDeviceToHost));
     FILE *f = std::fopen(path, "wb");
     if (!f) { set_error("mqe_save_state: cannot open the file"); return MQE_EINVAL; }
     StateHeader h;
     std::memset(&h, 0, sizeof(h));
-    std::memcpy(h.magic, "MQESTAT1", 8);
-    h.L = e->L;
+    std::memcpy(h.magic, "MQESTAT2", 8);
+    h.L = L;
     h.d = e->p.d;
     h.precision = e->p.precision;
     h.tile32 = e->tile32 ? 1 : 0;
@@ -2154,27 +2362,49 @@ int mqe_save_state(mqe_engine *e, const char *path, long step) {
     h.prepared = e->prepared ? 1 : 0;
     h.prop_count = e->prop_count;
     h.step = step;
+    h.capacity = e->p.capacity;
+    h.nC = nC;
+    h.prop_draws = e->prop_draws;
+    h.pop_draws = e->pop_draws;
+    h.canon_every = e->canon_every;
     const size_t tsz = e->fp32 ? sizeof(float) : sizeof(double);
     std::vector<char> buf(tsz * (size_t)std::max(e->wflat, e->eflat));
-    std::vector<int> vd(e->L + 1);
+    std::vector<int> vd(L + 1);
+    std::vector<double> bond(nC + 1);
+    std::vector<float> plab((size_t)nC * 16);
     bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1;
+    auto fail = [&](const char *msg) { std::fclose(f); set_error(msg); return MQE_EDEVICE; };
     for (int b = 0; ok && b < n; ++b) {
         const size_t sl = (size_t)order[b];
+        const int slot = order[b];
+        ok = std::fwrite(&slot, sizeof(int), 1, f) == 1;
         const struct { void *base; long len; } parts[3] = {{e->d_phi, e->wflat}, {e->d_Renv, e->eflat},
                                                            {e->d_Lenv, e->eflat}};
         for (const auto &pt : parts) {
             if (hipMemcpy(buf.data(), (char *)pt.base + tsz * sl * pt.len, tsz * pt.len, hipMemcpyDeviceToHost) !=
-                hipSuccess) { std::fclose(f); set_error("mqe_save_state: device copy failed"); return MQE_EDEVICE; }
+                hipSuccess)
+                return fail("mqe_save_state: device copy failed");
             ok = ok && std::fwrite(buf.data(), tsz, pt.len, f) == (size_t)pt.len;
         }
         double sc[4];
-        HIPCHK(hipMemcpy(vd.data(), e->d_wvd + sl * (e->L + 1), sizeof(int) * (e->L + 1), hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(&sc[0], e->d_w + sl, sizeof(double), hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(&sc[1], e->d_O + sl, sizeof(double), hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(&sc[2], e->d_eh + sl, sizeof(double), hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(&sc[3], e->d_EL + sl, sizeof(double), hipMemcpyDeviceToHost));
-        ok = ok && std::fwrite(vd.data(), sizeof(int), e->L + 1, f) == (size_t)(e->L + 1);
+        const bool cp = hipMemcpy(vd.data(), e->d_wvd + sl * (L + 1), sizeof(int) * (L + 1), hipMemcpyDeviceToHost) ==
+                            hipSuccess &&
+                        hipMemcpy(&sc[0], e->d_w + sl, sizeof(double), hipMemcpyDeviceToHost) == hipSuccess &&
+                        hipMemcpy(&sc[1], e->d_O + sl, sizeof(double), hipMemcpyDeviceToHost) == hipSuccess &&
+                        hipMemcpy(&sc[2], e->d_eh + sl, sizeof(double), hipMemcpyDeviceToHost) == hipSuccess &&
+                        hipMemcpy(&sc[3], e->d_EL + sl, sizeof(double), hipMemcpyDeviceToHost) == hipSuccess;
+        if (!cp) return fail("mqe_save_state: device copy failed");
+        ok = ok && std::fwrite(vd.data(), sizeof(int), L + 1, f) == (size_t)(L + 1);
         ok = ok && std::fwrite(sc, sizeof(double), 4, f) == 4;
+        if (e->tile32) {
+            if (hipMemcpy(bond.data(), e->d_bond + sl * (nC + 1), sizeof(double) * (nC + 1), hipMemcpyDeviceToHost) !=
+                    hipSuccess ||
+                hipMemcpy(plab.data(), e->d_plab + sl * nC * 16, sizeof(float) * nC * 16, hipMemcpyDeviceToHost) !=
+                    hipSuccess)
+                return fail("mqe_save_state: device copy failed");
+            ok = ok && std::fwrite(bond.data(), sizeof(double), nC + 1, f) == (size_t)(nC + 1);
+            ok = ok && std::fwrite(plab.data(), sizeof(float), (size_t)nC * 16, f) == (size_t)nC * 16;
+        }
     }
     ok = (std::fclose(f) == 0) && ok;
     if (!ok) { set_error("mqe_save_state: write failed"); return MQE_EINVAL; }
@@ -2186,13 +2416,14 @@ int mqe_load_state(mqe_engine *e, const char *path, long *step) {
     FILE *f = std::fopen(path, "rb");
     if (!f) { set_error("mqe_load_state: cannot open the file"); return MQE_EINVAL; }
     StateHeader h;
-    if (std::fread(&h, sizeof(h), 1, f) != 1 || std::memcmp(h.magic, "MQESTAT1", 8) != 0) {
+    if (std::fread(&h, sizeof(h), 1, f) != 1 || std::memcmp(h.magic, "MQESTAT2", 8) != 0) {
         std::fclose(f);
         set_error("mqe_load_state: not a walker-state file");
         return MQE_EINVAL;
     }
-    if (h.L != e->L || h.d != e->p.d || h.precision != e->p.precision || h.tile32 != (e->tile32 ? 1 : 0) ||
-        h.wflat != e->wflat || h.eflat != e->eflat) {
+    const int L = e->L, nC = e->nC;
+    if (h.L != L || h.d != e->p.d || h.precision != e->p.precision || h.tile32 != (e->tile32 ? 1 : 0) ||
+        h.wflat != e->wflat || h.eflat != e->eflat || h.capacity != e->p.capacity || h.nC != nC) {
         std::fclose(f);
         set_error("mqe_load_state: the file was written by an engine with other parameters");
         return MQE_EINVAL;
@@ -2203,41 +2434,72 @@ int mqe_load_state(mqe_engine *e, const char *path, long *step) {
         return MQE_ECAPACITY;
     }
     HIPCHK(hipStreamSynchronize(e->stream));
+    {
+        const int r = init_tile_env(e);  // boundary environments of every slot
+        if (r != MQE_OK) { std::fclose(f); return r; }
+    }
     const size_t tsz = e->fp32 ? sizeof(float) : sizeof(double);
     std::vector<char> buf(tsz * (size_t)std::max(e->wflat, e->eflat));
-    std::vector<int> vd(e->L + 1);
-    bool ok = true;
-    for (int b = 0; ok && b < h.n; ++b) {
-        const size_t sl = (size_t)b;
+    std::vector<int> vd(L + 1), order(e->p.capacity, -1);
+    std::vector<char> used(e->p.capacity, 0);
+    std::vector<double> bond(nC + 1);
+    std::vector<float> plab((size_t)nC * 16);
+    bool ok = true, dev = true;
+    for (int b = 0; ok && dev && b < h.n; ++b) {
+        int slot = -1;
+        ok = std::fread(&slot, sizeof(int), 1, f) == 1 && slot >= 0 && slot < e->p.capacity && !used[slot];
+        if (!ok) break;
+        used[slot] = 1;
+        order[b] = slot;
+        const size_t sl = (size_t)slot;
         const struct { void *base; long len; } parts[3] = {{e->d_phi, e->wflat}, {e->d_Renv, e->eflat},
                                                            {e->d_Lenv, e->eflat}};
         for (const auto &pt : parts) {
             ok = ok && std::fread(buf.data(), tsz, pt.len, f) == (size_t)pt.len;
             if (!ok) break;
-            HIPCHK(hipMemcpy((char *)pt.base + tsz * sl * pt.len, buf.data(), tsz * pt.len, hipMemcpyHostToDevice));
+            dev = dev && hipMemcpy((char *)pt.base + tsz * sl * pt.len, buf.data(), tsz * pt.len,
+                                   hipMemcpyHostToDevice) == hipSuccess;
         }
         double sc[4];
-        ok = ok && std::fread(vd.data(), sizeof(int), e->L + 1, f) == (size_t)(e->L + 1);
+        ok = ok && std::fread(vd.data(), sizeof(int), L + 1, f) == (size_t)(L + 1);
         ok = ok && std::fread(sc, sizeof(double), 4, f) == 4;
         if (!ok) break;
-        for (int k = 0; k <= e->L; ++k)
-            if (vd[k] < 1 || vd[k] > std::max(e->Dw, 1) || (e->tile32 && vd[k] > 32)) ok = false;
+        for (int k = 0; k <= L; ++k)
+            if (vd[k] < 1 || vd[k] > std::max(e->wmax[k], 1)) ok = false;
         if (!ok) break;
-        HIPCHK(hipMemcpy(e->d_wvd + sl * (e->L + 1), vd.data(), sizeof(int) * (e->L + 1), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(e->d_w + sl, &sc[0], sizeof(double), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(e->d_O + sl, &sc[1], sizeof(double), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(e->d_eh + sl, &sc[2], sizeof(double), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(e->d_EL + sl, &sc[3], sizeof(double), hipMemcpyHostToDevice));
+        dev = dev && hipMemcpy(e->d_wvd + sl * (L + 1), vd.data(), sizeof(int) * (L + 1), hipMemcpyHostToDevice) ==
+                         hipSuccess;
+        double *dst[4] = {e->d_w, e->d_O, e->d_eh, e->d_EL};
+        for (int q = 0; q < 4; ++q)
+            dev = dev && hipMemcpy(dst[q] + sl, &sc[q], sizeof(double), hipMemcpyHostToDevice) == hipSuccess;
+        if (e->tile32) {
+            ok = ok && std::fread(bond.data(), sizeof(double), nC + 1, f) == (size_t)(nC + 1);
+            ok = ok && std::fread(plab.data(), sizeof(float), (size_t)nC * 16, f) == (size_t)nC * 16;
+            if (!ok) break;
+            dev = dev && hipMemcpy(e->d_bond + sl * (nC + 1), bond.data(), sizeof(double) * (nC + 1),
+                                   hipMemcpyHostToDevice) == hipSuccess;
+            dev = dev && hipMemcpy(e->d_plab + sl * nC * 16, plab.data(), sizeof(float) * nC * 16,
+                                   hipMemcpyHostToDevice) == hipSuccess;
+        }
     }
     std::fclose(f);
+    if (!dev) { set_error("mqe_load_state: device copy failed"); return MQE_EDEVICE; }
     if (!ok) { set_error("mqe_load_state: truncated or inconsistent file"); return MQE_EINVAL; }
-    std::vector<int> ord(e->p.capacity);
-    for (int i = 0; i < e->p.capacity; ++i) ord[i] = i;
-    HIPCHK(hipMemcpy(e->d_order, ord.data(), sizeof(int) * ord.size(), hipMemcpyHostToDevice));
+    // the remaining slots follow in slot order (order beyond n is never read)
+    for (int s2 = 0, b = h.n; s2 < e->p.capacity; ++s2)
+        if (!used[s2]) order[b++] = s2;
+    HIPCHK(hipMemcpy(e->d_order, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(e->d_n, &h.n, sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(hipMemset(e->d_status, 0, sizeof(int)));
     e->n_host = h.n;
     e->prop_count = h.prop_count;
+    e->prop_draws = h.prop_draws;
+    e->pop_draws = h.pop_draws;
+    e->canon_every = h.canon_every;
+    e->ctl_host.step = (int)h.prop_draws;
+    e->ctl_host.pstep = (int)h.pop_draws;
+    e->ctl_host.pc = h.prop_count;
+    ECHK(push_ctl(e));
     e->loaded = true;
     e->prepared = h.prepared != 0;
     if (step) *step = h.step;

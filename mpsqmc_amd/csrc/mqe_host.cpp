@@ -352,6 +352,46 @@ int mqe_left_normalize(int L, int d, const int *vd, double *data, double *norm) 
     return MQE_OK;
 }
 
+int mqe_right_normalize(int L, int d, const int *vd, double *data, double *norm) {
+    if (L < 1 || d < 1 || !vd || !data) { mqe::set_error("mqe_right_normalize: bad arguments"); return MQE_EINVAL; }
+    HostMps m;
+    m.load(L, d, vd, data);
+    double last = 1.0;
+    for (int k = L - 1; k >= 0; --k) {
+        // LQ of the site as a dimL x (d dimR) matrix = QR of its transpose,
+        // rows (col + dimR*s)
+        const int dl = m.vd[k], dr = m.vd[k + 1], cols = dr * d;
+        if (cols < dl) { mqe::set_error("mqe_right_normalize: d*dimR < dimL"); return MQE_EINVAL; }
+        std::vector<double> a((size_t)cols * dl), r((size_t)dl * dl);
+        for (int s = 0; s < d; ++s)
+            for (int c = 0; c < dr; ++c)
+                for (int rr = 0; rr < dl; ++rr) a[(c + (size_t)dr * s) + (size_t)cols * rr] = m.t[k][rr + (size_t)dl * (c + (size_t)dr * s)];
+        qr_householder(cols, dl, a.data(), r.data());
+        for (int s = 0; s < d; ++s)
+            for (int c = 0; c < dr; ++c)
+                for (int rr = 0; rr < dl; ++rr) m.t[k][rr + (size_t)dl * (c + (size_t)dr * s)] = a[(c + (size_t)dr * s) + (size_t)cols * rr];
+        if (k > 0) {
+            // left neighbour times Lmat = R^T: B[s](a, q) <- sum_p B[s](a, p) R(q, p)
+            const int dp = m.vd[k - 1];
+            std::vector<double> nt((size_t)dp * dl * d, 0.0);
+            for (int s = 0; s < d; ++s)
+                for (int q = 0; q < dl; ++q)
+                    for (int p = q; p < dl; ++p) {
+                        const double rv = r[q + (size_t)dl * p];
+                        if (rv == 0.0) continue;
+                        for (int aa = 0; aa < dp; ++aa)
+                            nt[aa + (size_t)dp * (q + (size_t)dl * s)] += m.t[k - 1][aa + (size_t)dp * (p + (size_t)dl * s)] * rv;
+                    }
+            m.t[k - 1].swap(nt);
+        } else {
+            last = r[0];
+        }
+    }
+    m.store(data);
+    if (norm) *norm = last;
+    return MQE_OK;
+}
+
 int mqe_compress(int L, int d, const int *vd, const double *data, int Dt, int *vd_out, double *out) {
     if (L < 2 || d < 1 || Dt < 1 || !vd || !data || !vd_out || !out) {
         mqe::set_error("mqe_compress: bad arguments");

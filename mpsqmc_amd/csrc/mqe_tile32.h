@@ -623,10 +623,12 @@ template <bool LEARN>
 __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, const float *TN, const float *TT,
                                                               int flags) {
     extern __shared__ float lds32[];
-    const int b = blockIdx.x;
     const int n = *P.n;
-    if (b >= n) return;
+    if ((int)blockIdx.x >= n) return;
+    if (*P.status & 4) return;  // the population overflowed: keep the state as it was
+    const int b = P.list ? P.list[blockIdx.x] : (int)blockIdx.x;  // population position
     const int slot = P.order[b];
+    const Ctl ctl = *P.ctl;
     const int L = S.L;
     __shared__ int wv[MQE_MAXL + 1];
     __shared__ double red[2 * MQE_WAVES];
@@ -853,7 +855,7 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
             ln += pv[t];
         }
         const double u = P.u ? P.u[(size_t)b * S.nC + c]
-                             : philox_uniform(P.seed, (uint32_t)b, (uint32_t)c, (uint32_t)P.step, 0u);
+                             : philox_uniform(P.seed, (uint32_t)b, (uint32_t)c, (uint32_t)ctl.step, 0u);
         double run = wave_exclusive_scan(ln), cnt = 0.0;
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
@@ -1011,7 +1013,7 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
     __syncthreads();
     // ---------------- local energy (Walker.cpp:72-77) ----------------
     double On, Ou;
-    const double e = energy_sweep32(C, pc, &On, &Ou, (flags & F_CANON) != 0);
+    const double e = energy_sweep32(C, pc, &On, &Ou, canon_slot(ctl, slot));
     if (threadIdx.x == 0) {
         P.w[slot] = wt;
         P.O[slot] = On;

@@ -221,10 +221,13 @@ class Engine:
         return dict(w=w, O=O, eh=eh, EL=EL)
 
     def indices(self, n=None):
-        n = self.n if n is None else n
-        idx = np.zeros(n * self.nC, dtype=np.int32)
+        """Sampled grid points of the last propagation (walkers x couplings); n: the first n walkers."""
+        last = self.last_kernel_ms()[1]
+        idx = np.zeros(max(last, 1) * self.nC, dtype=np.int32)
         check(lib().mqe_read_indices(self.h, _iptr(idx)))
-        return idx.reshape(n, self.nC)
+        n = last if n is None else n
+        assert n <= last, (n, last)
+        return idx[: n * self.nC].reshape(n, self.nC)
 
     def walker(self, i):
         vd = np.zeros(self.L + 1, dtype=np.int32)

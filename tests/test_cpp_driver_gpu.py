@@ -59,3 +59,32 @@ def test_cpp_walk_compressed_walkers(tmp_path):
     E0o, tr = pop.walk(3, 5, 8, 1000)
     assert abs(E0 - E0o) < 1e-9 * abs(E0o)
     assert np.allclose(rows[:, 2], tr[:, 2], rtol=1e-9)
+
+
+def test_cpp_device_rng_walk_follows_the_seed(tmp_path):
+    # MPSQMC2 with Rng::Device keys the device random numbers with the caller's
+    # seed: its call-by-call walk (propagate, energy, population control) equals
+    # mqe_run on the same trial and seed, and another seed walks differently
+    from mpsqmc_amd.engine import Engine
+    s = systems.chain(10)
+    vd, data = random_trial(10, 6, 47)
+    path = str(tmp_path / "trial.mps")
+    M.write_mps(path, vd, data)
+    N, steps = 16, 5
+    rows = {}
+    for seed in (101, 202):
+        E0, r = _run(["--trial", path, "--N", str(N), "--steps", str(steps), "--seed", str(seed), "--device-rng"])
+        rows[seed] = r
+    # the driver left-normalises the trial it reads (MPSQMC2.cpp:239): same routine here
+    _, _, _, tv, td = M.read_mps(path)
+    td, _ = M.left_normalize(tv, td)
+    eng = Engine(s, tv, td, n_desired=N, precision="fp64", seed=101)
+    eng.load_copies(tv, td, N)
+    eng.prepare()
+    eng.run(1, steps)
+    tr = eng.read_trace(1, steps)
+    eng.close()
+    assert np.array_equal(rows[101][:, 1], tr[:, 1])
+    for col in (2, 3, 4, 5):
+        assert np.allclose(rows[101][:, col], tr[:, col], rtol=1e-12, atol=1e-14), col
+    assert not np.allclose(rows[202][:, 5], tr[:, 5], rtol=1e-9), "the device draws ignore the seed"

@@ -351,6 +351,61 @@ def test_propagate_matches_dense_brute_force():
     assert np.allclose(M.to_dense(vdo, do), phi, atol=1e-11)
 
 
+def test_propagate_with_field_matches_dense_brute_force():
+    # h != 0: PropagateSeparately applies exp(dtau h S^z / 2) on every site before
+    # and after the couplings, each followed by setOverlap (MPSQMC2.cpp:532-534,
+    # 596-598; TrotterHeisenberg.cpp:80-88 for H = ... - h sum S^z).  The dense
+    # restatement builds that factor with scipy's expm of the field term of the
+    # dense Hamiltonian, and E_L = <trial|H|phi>/<trial|phi> with the dense H.
+    import scipy.linalg as sl
+    from .common import dense_hamiltonian
+    L, h = 6, 0.7
+    sysm = systems.chain(L, h=h)
+    tvd, tdata = random_trial(L, 4, 23)
+    trial = M.to_dense(tvd, tdata)
+    pop = oracle_population(sysm, tvd, tdata)
+    wvd, wdata = random_trial(L, 4, 24)
+    pop.set_walkers([wvd], [wdata], np.array([1.0]), np.array([1.0]), np.array([0.0]))
+    u = np.random.default_rng(5).uniform(size=pop.nC)
+    sw, idx = pop.propagate(u)
+    H = dense_hamiltonian(sysm, sparse=False)
+    H0 = dense_hamiltonian(systems.chain(L), sparse=False)
+    half = sl.expm(-0.5 * sysm.dtau * (H - H0))  # exp(dtau h/2 sum S^z)
+    assert np.allclose(np.diag(np.diag(half)), half)
+    grid = sysm.grid()
+    S, U, VT = sysm.tables_per_coupling()
+    f, s, _ = sysm.couplings()
+
+    def normalise(v):
+        v = v / np.linalg.norm(v)
+        O = trial @ v
+        return (-v, -O) if O < 0 else (v, O)
+
+    phi, O = normalise(half @ M.to_dense(wvd, wdata))
+    w = 1.0
+    for c in range(pop.nC):
+        num = np.zeros(len(grid))
+        for g, x in enumerate(grid):
+            Ux, Vx = _dense_ops(S[c], U[c], VT[c], x)
+            num[g] = trial @ (_op_on(L, f[c], Ux) @ (_op_on(L, s[c], Vx) @ phi))
+        pdf = np.maximum(0.0, num) / (O * len(grid))
+        norm = pdf.sum()
+        w *= norm
+        cum = np.cumsum(pdf / norm)
+        gsel = min(int(np.sum(cum[:-1] <= u[c])), len(grid) - 1)
+        assert gsel == idx[0, c]
+        Ux, Vx = _dense_ops(S[c], U[c], VT[c], grid[gsel])
+        phi, O = normalise(_op_on(L, f[c], Ux) @ (_op_on(L, s[c], Vx) @ phi))
+    phi, O = normalise(half @ phi)
+    wo, Oo, _ = pop.scalars()
+    assert abs(wo[0] - w) <= 1e-11 * w
+    assert abs(Oo[0] - O) <= 1e-11 * O
+    vdo, do, *_ = pop.walker(0)
+    assert np.allclose(M.to_dense(vdo, do), phi, atol=1e-11)
+    EP, _, EL = pop.energy(1, False)
+    assert abs(EL[0] - (trial @ (H @ phi)) / O) <= 1e-10 * max(1.0, abs(EL[0]))
+
+
 def test_energy_exact_trial_is_eigenvalue():
     # E_L == E0 for every walker when the trial is an exact eigenstate
     L = 8
