@@ -118,6 +118,39 @@ int mqe_run(mqe_engine *e, int first_step, int n_steps, int use_graph);
 /* Copy the trace rows of steps first_step .. first_step+n_steps-1 (6 doubles each). */
 int mqe_read_trace(mqe_engine *e, int first_step, int n_steps, double *out);
 
+/* MPSstate::CompressState(targets[i]) (MPSstate.cpp:379-404) for every walker i
+ * of the population on the device, followed by Walker::setOverlap
+ * (Walker.cpp:54-66), as SetupWalkers does for DW < DT (MPSQMC2.cpp:402-409):
+ * a left-to-right and a right-to-left sweep of FP64 two-site SVDs (one-sided
+ * Jacobi) keeping min(targets[i], number of Schmidt values >= 1e-13) states per
+ * bond (TwoSiteObject.cpp:16,119-184), the state normalised, its phase chosen so
+ * that the overlap with the trial is positive, and that overlap stored.
+ * targets[i] < 1 leaves walker i (and its overlap) unchanged.  Call
+ * mqe_prepare afterwards. */
+int mqe_compress_walkers(mqe_engine *e, const int *targets);
+
+/* ---- tile shapes of the general path ----------------------------------------
+ * No counterpart in the reference (its contractions are BLAS calls on one
+ * walker at a time).  The general path's FP32 products run on the matrix cores
+ * with one of three output tile shapes per wave: 16 x 16 (v_mfma_f32_16x16x4),
+ * 32 x 32 or 32 x 64 (two 32 x 32 tiles sharing their A panel).  The shape is
+ * chosen per walker by its bond bucket (largest bond 1-16, 17-32, ..., > 112)
+ * from an 8-entry table: tiles[bucket] in {16, 32, 64}.  mqe_create takes the
+ * table tuned for the engine's key (L, couplings, trial and walker bonds,
+ * precision) from the cache file ($MQE_TUNE_CACHE, else
+ * ~/.cache/mqe_tiles.txt) when it has one.  Results do not depend on the
+ * table beyond FP32 rounding; FP64 engines and the tile path ignore it. */
+int mqe_get_tiles(mqe_engine *e, int *tiles);
+int mqe_set_tiles(mqe_engine *e, const int *tiles);
+/* Time `reps` propagations of the current population with each candidate
+ * shape of each bond bucket present (coordinate descent, largest bonds first;
+ * the population is restored after every trial), keep the fastest table,
+ * append it to the cache file and return it; *ms_default / *ms_tuned: walker
+ * kernel time of the starting and of the chosen table (either may be NULL).
+ * Needs mqe_prepare; a no-op (returning the table) on FP64 engines and the
+ * tile path. */
+int mqe_autotune(mqe_engine *e, int reps, int *tiles_out, double *ms_default, double *ms_tuned);
+
 /* 1 when the engine runs the FP32 MFMA tile path (nearest-neighbour chain,
  * all bonds <= 32, h = 0), 0 for the general path. */
 int mqe_kernel_path(mqe_engine *e);

@@ -96,6 +96,10 @@ SIGNATURES = {
     "mqe_set_proposal": (c_int, [ctypes.c_void_p, ctypes.c_void_p, c_double]),
     "mqe_set_proposal_table": (c_int, [ctypes.c_void_p, P(ctypes.c_float), c_int, c_double]),
     "mqe_read_bond_energies": (c_int, [ctypes.c_void_p, P(c_double)]),
+    "mqe_compress_walkers": (c_int, [ctypes.c_void_p, P(c_int)]),
+    "mqe_get_tiles": (c_int, [ctypes.c_void_p, P(c_int)]),
+    "mqe_set_tiles": (c_int, [ctypes.c_void_p, P(c_int)]),
+    "mqe_autotune": (c_int, [ctypes.c_void_p, c_int, P(c_int), P(c_double), P(c_double)]),
     "mqe_save_state": (c_int, [ctypes.c_void_p, ctypes.c_char_p, c_long]),
     "mqe_load_state": (c_int, [ctypes.c_void_p, ctypes.c_char_p, P(c_long)]),
     "mqe_bond_dims": (None, [c_int, c_int, c_int, P(c_int)]),
@@ -105,6 +109,9 @@ SIGNATURES = {
                          ctypes.c_uint, P(c_int), P(c_double), P(c_double)]),
     "mqe_left_normalize": (c_int, [c_int, c_int, P(c_int), P(c_double), P(c_double)]),
     "mqe_compress": (c_int, [c_int, c_int, P(c_int), P(c_double), c_int, P(c_int), P(c_double)]),
+    "mqe_right_normalize": (c_int, [c_int, c_int, P(c_int), P(c_double), P(c_double)]),
+    "mqe_apply_mpo": (c_int, [c_int, c_int, c_int, P(c_int), P(c_int), P(c_double), c_double, P(c_int), P(c_double),
+                              P(c_int), P(c_double), P(c_long)]),
     "mqe_rng_create": (c_int, [ctypes.c_uint, P(ctypes.c_void_p)]),
     "mqe_rng_destroy": (None, [ctypes.c_void_p]),
     "mqe_rng_uniforms": (c_int, [ctypes.c_void_p, c_long, P(c_double)]),

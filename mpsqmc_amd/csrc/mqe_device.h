@@ -213,16 +213,161 @@ __device__ __forceinline__ void gemm_mfma(int m, int n, int k, const float *A, i
 }
 
 // ---------------------------------------------------------------------------
+// The same product with 16 x 16 output tiles (v_mfma_f32_16x16x4f32): small
+// operands (bonds of 16 or 48) waste less of each tile and spread over all
+// four waves.  Within a 16-deep slice of k starting at k0, MFMA u takes
+// kk = k0 + 4(l/16) + u in lane l: lane l holds A(l % 16, kk) and B(kk, l % 16)
+// (four consecutive values of an operand stored along k: one 16-byte load),
+// and accumulator slot q holds element (4(l/16) + q, l % 16) of the tile.
+// ---------------------------------------------------------------------------
+template <bool TA, bool TB>
+__device__ __forceinline__ void gemm_mfma16(int m, int n, int k, const float *A, int lda, const float *B, int ldb,
+                                            float *C, int ldc, bool acc) {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int r = lane & 15, g = lane >> 4;
+    const int tm = (m + 15) >> 4, tn = (n + 15) >> 4;
+    const bool avec = TA && ((((uintptr_t)A) | ((uintptr_t)lda * 4u)) & 15u) == 0;
+    const bool bvec = !TB && ((((uintptr_t)B) | ((uintptr_t)ldb * 4u)) & 15u) == 0;
+    for (int t = w; t < tm * tn; t += MQE_WAVES) {
+        const int r0 = (t % tm) * 16, c0 = (t / tm) * 16;
+        const int ar = r0 + r, bc = c0 + r;
+        const bool aok = ar < m, bok = bc < n;
+        v4 c = {0.f, 0.f, 0.f, 0.f};
+        for (int k0 = 0; k0 < k; k0 += 16) {
+            const int kb = k0 + 4 * g;
+            const bool full = k0 + 16 <= k;
+            float a[4], b[4];
+            if (avec && full) {
+                const float4 x = *(const float4 *)(A + (size_t)lda * (aok ? ar : 0) + kb);
+                a[0] = aok ? x.x : 0.f; a[1] = aok ? x.y : 0.f; a[2] = aok ? x.z : 0.f; a[3] = aok ? x.w : 0.f;
+            } else {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int kk = kb + u;
+                    a[u] = (aok && kk < k) ? (TA ? A[kk + (size_t)lda * ar] : A[ar + (size_t)lda * kk]) : 0.f;
+                }
+            }
+            if (bvec && full) {
+                const float4 x = *(const float4 *)(B + (size_t)ldb * (bok ? bc : 0) + kb);
+                b[0] = bok ? x.x : 0.f; b[1] = bok ? x.y : 0.f; b[2] = bok ? x.z : 0.f; b[3] = bok ? x.w : 0.f;
+            } else {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int kk = kb + u;
+                    b[u] = (bok && kk < k) ? (TB ? B[bc + (size_t)ldb * kk] : B[kk + (size_t)ldb * bc]) : 0.f;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], b[u], c, 0, 0, 0);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int rr = r0 + 4 * g + q;
+            if (rr < m && bok) {
+                float v = c[q];
+                if (acc) v += C[rr + (size_t)ldc * bc];
+                C[rr + (size_t)ldc * bc] = v;
+            }
+        }
+    }
+}
+
+// 32 x 64 output blocks: each wave computes two neighbouring 32 x 32 tiles that
+// share the A panel (half the A reads of gemm_mfma for wide products).
+template <bool TA, bool TB>
+__device__ __forceinline__ void gemm_mfma_wide(int m, int n, int k, const float *A, int lda, const float *B,
+                                               int ldb, float *C, int ldc, bool acc) {
+    typedef float v16 __attribute__((ext_vector_type(16)));
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int r = lane & 31, h = lane >> 5;
+    const int tm = (m + 31) >> 5, tn = (n + 63) >> 6;
+    const bool avec = TA && ((((uintptr_t)A) | ((uintptr_t)lda * 4u)) & 15u) == 0;
+    const bool bvec = !TB && ((((uintptr_t)B) | ((uintptr_t)ldb * 4u)) & 15u) == 0;
+    for (int t = w; t < tm * tn; t += MQE_WAVES) {
+        const int r0 = (t % tm) * 32, c0 = (t / tm) * 64;
+        const bool two = c0 + 32 < n;  // wave-uniform
+        const int ar = r0 + r;
+        const bool aok = ar < m;
+        v16 c[2];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) { c[0][q] = 0.f; c[1][q] = 0.f; }
+        for (int k0 = 0; k0 < k; k0 += 16) {
+            const int kb = k0 + 8 * h;
+            const bool full = k0 + 16 <= k;
+            float a[8];
+            if (avec && full) {
+                const float4 *p = (const float4 *)(A + (size_t)lda * (aok ? ar : 0) + kb);
+                const float4 x = p[0], y = p[1];
+                a[0] = aok ? x.x : 0.f; a[1] = aok ? x.y : 0.f; a[2] = aok ? x.z : 0.f; a[3] = aok ? x.w : 0.f;
+                a[4] = aok ? y.x : 0.f; a[5] = aok ? y.y : 0.f; a[6] = aok ? y.z : 0.f; a[7] = aok ? y.w : 0.f;
+            } else {
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int kk = kb + u;
+                    a[u] = (aok && kk < k) ? (TA ? A[kk + (size_t)lda * ar] : A[ar + (size_t)lda * kk]) : 0.f;
+                }
+            }
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                if (half == 1 && !two) break;
+                const int bc = c0 + 32 * half + r;
+                const bool bok = bc < n;
+                float b[8];
+                if (bvec && full) {
+                    const float4 *p = (const float4 *)(B + (size_t)ldb * (bok ? bc : 0) + kb);
+                    const float4 x = p[0], y = p[1];
+                    b[0] = bok ? x.x : 0.f; b[1] = bok ? x.y : 0.f; b[2] = bok ? x.z : 0.f; b[3] = bok ? x.w : 0.f;
+                    b[4] = bok ? y.x : 0.f; b[5] = bok ? y.y : 0.f; b[6] = bok ? y.z : 0.f; b[7] = bok ? y.w : 0.f;
+                } else {
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int kk = kb + u;
+                        b[u] = (bok && kk < k) ? (TB ? B[bc + (size_t)ldb * kk] : B[kk + (size_t)ldb * bc]) : 0.f;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) c[half] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u], b[u], c[half], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            if (half == 1 && !two) break;
+            const int bc = c0 + 32 * half + r;
+            if (bc >= n) continue;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int rr = r0 + 8 * (q >> 2) + 4 * h + (q & 3);
+                if (rr < m) {
+                    float v = c[half][q];
+                    if (acc) v += C[rr + (size_t)ldc * bc];
+                    C[rr + (size_t)ldc * bc] = v;
+                }
+            }
+        }
+    }
+}
+
+// Output tile shapes of the FP32 matrix-core products (chosen per walker bond
+// bucket by the engine's tile table).
+enum : int { TILE_16 = 16, TILE_32 = 32, TILE_WIDE = 64 };
+
+// ---------------------------------------------------------------------------
 // GEMM: C(m x n) (+)= op(A) op(B).  op(A) is m x k: A(r,q) = TA ? A[q + lda*r] : A[r + lda*q].
-// FP32 products of useful size run on the matrix cores, the rest (and all of
-// FP64) on the vector units.
+// FP32 products of useful size run on the matrix cores with the output tile
+// shape `tile`, the rest (and all of FP64) on the vector units.
 // ---------------------------------------------------------------------------
 template <bool TA, bool TB, typename T>
 __device__ __forceinline__ void gemm(int m, int n, int k, const T *A, int lda, const T *B, int ldb, T *C, int ldc,
-                                     bool acc) {
+                                     bool acc, int tile = TILE_32) {
     if constexpr (sizeof(T) == sizeof(float)) {
         if (m >= 8 && n >= 8 && k >= 4) {
-            gemm_mfma<TA, TB>(m, n, k, (const float *)A, lda, (const float *)B, ldb, (float *)C, ldc, acc);
+            if (tile == TILE_16)
+                gemm_mfma16<TA, TB>(m, n, k, (const float *)A, lda, (const float *)B, ldb, (float *)C, ldc, acc);
+            else if (tile == TILE_WIDE && n > 32)
+                gemm_mfma_wide<TA, TB>(m, n, k, (const float *)A, lda, (const float *)B, ldb, (float *)C, ldc, acc);
+            else
+                gemm_mfma<TA, TB>(m, n, k, (const float *)A, lda, (const float *)B, ldb, (float *)C, ldc, acc);
             return;
         }
     }

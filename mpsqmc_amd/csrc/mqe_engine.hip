@@ -34,9 +34,12 @@
 #include <string>
 #include <vector>
 
+#include <sys/stat.h>
+
 #include "mqe.h"
 #include "mqe_device.h"
 #include "mqe_internal.h"
+#include "mqe_svd.h"
 
 #define MQE_MAXL 256
 #define MQE_MAXG 4096
@@ -82,7 +85,11 @@ struct Sys {
     double Sz[2];
     double h;
     double ssp[2];                   // exp(h dtau m / 2)
+    int tiles[8];                    // FP32 product tile shape by walker bond bucket (max bond 1-16, 17-32, ...)
 };
+
+// bond bucket of a walker: its largest bond in steps of 16 (0: <= 16, ..., 7: > 112)
+__host__ __device__ inline int bond_bucket(int maxbond) { return min(7, max(0, (maxbond - 1) / 16)); }
 
 // The walker population (device pointers).
 struct Pool {
@@ -173,6 +180,7 @@ struct Ctx {
     double *red;         // LDS reduction scratch (>= MQE_NT + 16*MQE_WAVES)
     double *qlds;        // LDS workspace of the LQ (tau, then trailing columns), or nullptr
     long qlds_doubles;   // its size
+    int tile;            // output tile shape of the FP32 matrix-core products (TILE_*)
 
     __device__ T *site(int k) const { return phi + woff[k]; }
     __device__ const T *tsite(int k) const { return trial + S->toff[k]; }
@@ -187,9 +195,9 @@ template <typename T>
 __device__ void compute_G(const Ctx<T> &C, int i, const T *Lenv_i) {
     const int ti = C.tv[i], tn = C.tv[i + 1], wi = C.wv[i], wn = C.wv[i + 1];
     T *Y = C.ts + C.sl.Y, *G = C.ts + C.sl.G;
-    gemm<false, false, T>(ti, 2 * wn, wi, Lenv_i, ti, C.site(i), wi, Y, ti, false);
+    gemm<false, false, T>(ti, 2 * wn, wi, Lenv_i, ti, C.site(i), wi, Y, ti, false, C.tile);
     __syncthreads();
-    gemm<true, false, T>(2 * tn, 2 * wn, ti, C.tsite(i), ti, Y, ti, G, 2 * tn, false);
+    gemm<true, false, T>(2 * tn, 2 * wn, ti, C.tsite(i), ti, Y, ti, G, 2 * tn, false, C.tile);
     __syncthreads();
 }
 
@@ -201,12 +209,12 @@ __device__ void compute_V(const Ctx<T> &C, int j) {
     T *W = C.ts + C.sl.W, *V = C.ts + C.sl.V;
     const T *R = C.Rk(j + 1), *ph = C.site(j);
     for (int t = 0; t < 2; ++t)
-        gemm<false, true, T>(tn, wj, wn, R, tn, ph + (size_t)t * wj * wn, wj, W + (size_t)t * tn * wj, tn, false);
+        gemm<false, true, T>(tn, wj, wn, R, tn, ph + (size_t)t * wj * wn, wj, W + (size_t)t * tn * wj, tn, false, C.tile);
     __syncthreads();
     const T *Tj = C.tsite(j);
     for (int tp = 0; tp < 2; ++tp)
         gemm<false, false, T>(tj, 2 * wj, tn, Tj + (size_t)tp * tj * tn, tj, W, tn, V + (size_t)tp * tj * 2 * wj, tj,
-                              false);
+                              false, C.tile);
     __syncthreads();
 }
 
@@ -218,13 +226,13 @@ __device__ void carry_channels(const Ctx<T> &C, int k, const T *src, T *dst) {
     T *Y = C.ts + C.sl.Y;
     const T *Tk = C.tsite(k);
     for (int ch = 0; ch < 4; ++ch) {
-        gemm<false, false, T>(tk, 2 * wn, wk, src + (size_t)ch * tk * wk, tk, C.site(k), wk, Y, tk, false);
+        gemm<false, false, T>(tk, 2 * wn, wk, src + (size_t)ch * tk * wk, tk, C.site(k), wk, Y, tk, false, C.tile);
         __syncthreads();
         T *d = dst + (size_t)ch * tn * wn;
-        gemm<true, false, T>(tn, wn, tk, Tk, tk, Y, tk, d, tn, false);
+        gemm<true, false, T>(tn, wn, tk, Tk, tk, Y, tk, d, tn, false, C.tile);
         // no barrier: the accumulating product has the same shape, so every
         // thread re-reads only the output elements it wrote itself
-        gemm<true, false, T>(tn, wn, tk, Tk + (size_t)tk * tn, tk, Y + (size_t)tk * wn, tk, d, tn, true);
+        gemm<true, false, T>(tn, wn, tk, Tk + (size_t)tk * tn, tk, Y + (size_t)tk * wn, tk, d, tn, true, C.tile);
         __syncthreads();
     }
 }
@@ -305,13 +313,13 @@ template <typename T>
 __device__ void advance_L(const Ctx<T> &C, int k) {
     const int tk = C.tv[k], tn = C.tv[k + 1], wk = C.wv[k], wn = C.wv[k + 1];
     T *Y = C.ts + C.sl.Y;
-    gemm<false, false, T>(tk, 2 * wn, wk, C.Lk(k), tk, C.site(k), wk, Y, tk, false);
+    gemm<false, false, T>(tk, 2 * wn, wk, C.Lk(k), tk, C.site(k), wk, Y, tk, false, C.tile);
     __syncthreads();
     const T *Tk = C.tsite(k);
     T *Ln = C.Lk(k + 1);
-    gemm<true, false, T>(tn, wn, tk, Tk, tk, Y, tk, Ln, tn, false);
+    gemm<true, false, T>(tn, wn, tk, Tk, tk, Y, tk, Ln, tn, false, C.tile);
     // same-shape accumulation: no barrier needed (each thread owns its outputs)
-    gemm<true, false, T>(tn, wn, tk, Tk + (size_t)tk * tn, tk, Y + (size_t)tk * wn, tk, Ln, tn, true);
+    gemm<true, false, T>(tn, wn, tk, Tk + (size_t)tk * tn, tk, Y + (size_t)tk * wn, tk, Ln, tn, true, C.tile);
     __syncthreads();
 }
 
@@ -321,11 +329,11 @@ __device__ void advance_N(const Ctx<T> &C, int k, const T *N, T *Nn) {
     const int wk = C.wv[k], wn = C.wv[k + 1];
     T *X = C.ts + C.sl.Y;
     const T *ph = C.site(k);
-    gemm<false, false, T>(wk, 2 * wn, wk, N, wk, ph, wk, X, wk, false);
+    gemm<false, false, T>(wk, 2 * wn, wk, N, wk, ph, wk, X, wk, false, C.tile);
     __syncthreads();
-    gemm<true, false, T>(wn, wn, wk, ph, wk, X, wk, Nn, wn, false);
+    gemm<true, false, T>(wn, wn, wk, ph, wk, X, wk, Nn, wn, false, C.tile);
     // same-shape accumulation: no barrier needed (each thread owns its outputs)
-    gemm<true, false, T>(wn, wn, wk, ph + (size_t)wk * wn, wk, X + (size_t)wk * wn, wk, Nn, wn, true);
+    gemm<true, false, T>(wn, wn, wk, ph + (size_t)wk * wn, wk, X + (size_t)wk * wn, wk, Nn, wn, true, C.tile);
     __syncthreads();
 }
 
@@ -335,7 +343,7 @@ __device__ double norm2_at(const Ctx<T> &C, int k, const T *N) {
     const int wk = C.wv[k], wn = C.wv[k + 1];
     T *Z = C.ts + C.sl.Y;
     const T *ph = C.site(k);
-    gemm<false, false, T>(wk, 2 * wn, wk, N, wk, ph, wk, Z, wk, false);
+    gemm<false, false, T>(wk, 2 * wn, wk, N, wk, ph, wk, Z, wk, false, C.tile);
     __syncthreads();
     double s = 0.0;
     for (int e = threadIdx.x; e < wk * 2 * wn; e += MQE_NT) s += (double)ph[e] * (double)Z[e];
@@ -351,11 +359,11 @@ __device__ void advance_M(const Ctx<T> &C, int k) {
     const T *ph = C.site(k), *Mn = C.Mk(k + 1);
     T *Mk = C.Mk(k);
     for (int s = 0; s < 2; ++s)
-        gemm<false, false, T>(wk, wn, wn, ph + (size_t)s * wk * wn, wk, Mn, wn, X + (size_t)s * wk * wn, wk, false);
+        gemm<false, false, T>(wk, wn, wn, ph + (size_t)s * wk * wn, wk, Mn, wn, X + (size_t)s * wk * wn, wk, false, C.tile);
     __syncthreads();
-    gemm<false, true, T>(wk, wk, wn, X, wk, ph, wk, Mk, wk, false);
+    gemm<false, true, T>(wk, wk, wn, X, wk, ph, wk, Mk, wk, false, C.tile);
     // same-shape accumulation: no barrier needed (each thread owns its outputs)
-    gemm<false, true, T>(wk, wk, wn, X + (size_t)wk * wn, wk, ph + (size_t)wk * wn, wk, Mk, wk, true);
+    gemm<false, true, T>(wk, wk, wn, X + (size_t)wk * wn, wk, ph + (size_t)wk * wn, wk, Mk, wk, true, C.tile);
     __syncthreads();
 }
 
@@ -366,10 +374,10 @@ __device__ double norm2_at_M(const Ctx<T> &C, int k, const T *N) {
     const int wk = C.wv[k], wn = C.wv[k + 1];
     T *Z = C.ts + C.sl.Y, *Z2 = C.ts + C.sl.TMP;
     const T *ph = C.site(k), *Mn = C.Mk(k + 1);
-    gemm<false, false, T>(wk, 2 * wn, wk, N, wk, ph, wk, Z, wk, false);
+    gemm<false, false, T>(wk, 2 * wn, wk, N, wk, ph, wk, Z, wk, false, C.tile);
     __syncthreads();
     for (int s = 0; s < 2; ++s)
-        gemm<false, false, T>(wk, wn, wn, Z + (size_t)s * wk * wn, wk, Mn, wn, Z2 + (size_t)s * wk * wn, wk, false);
+        gemm<false, false, T>(wk, wn, wn, Z + (size_t)s * wk * wn, wk, Mn, wn, Z2 + (size_t)s * wk * wn, wk, false, C.tile);
     __syncthreads();
     double acc = 0.0;
     for (int e = threadIdx.x; e < wk * 2 * wn; e += MQE_NT) acc += (double)ph[e] * (double)Z2[e];
@@ -433,7 +441,7 @@ __device__ void lq_site(const Ctx<T> &C, int k) {
     // push Lmat into the left neighbour
     T *pl = C.site(k - 1), *tmp = C.ts + C.sl.TMP;
     for (int s = 0; s < 2; ++s)
-        gemm<false, false, T>(wp, wk, wk, pl + (size_t)s * wp * wk, wp, LM, wk, tmp + (size_t)s * wp * wk, wp, false);
+        gemm<false, false, T>(wp, wk, wk, pl + (size_t)s * wp * wk, wp, LM, wk, tmp + (size_t)s * wp * wk, wp, false, C.tile);
     __syncthreads();
     for (int e = threadIdx.x; e < 2 * wp * wk; e += MQE_NT) pl[e] = tmp[e];
     __syncthreads();
@@ -629,13 +637,21 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker_kernel(Sys S, Pool P, const 
     __shared__ double sh_scal[4];
     __shared__ int sh_g;
 
+    __shared__ int sh_tile;
     for (int k = threadIdx.x; k <= L; k += MQE_NT) {
         tv[k] = S.tvd[k];
         wv[k] = P.wvd[(size_t)slot * (L + 1) + k];
     }
     __syncthreads();
+    if (threadIdx.x == 0) {
+        int mb = 1;
+        for (int k = 0; k <= L; ++k) mb = max(mb, wv[k]);
+        sh_tile = S.tiles[bond_bucket(mb)];
+    }
+    __syncthreads();
 
     Ctx<T> C;
+    C.tile = sh_tile;
     C.S = &S;
     C.trial = trial;
     C.phi = (T *)P.phi + (size_t)slot * P.wflat;
@@ -830,8 +846,8 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker_kernel(Sys S, Pool P, const 
 // out[b] = <other | walker_b> (transfer matrices, MPSstate.cpp:248-284).
 template <typename T>
 __global__ __launch_bounds__(MQE_NT) void overlap_kernel(int L, const int *ovd, const long *ooff, const T *other,
-                                                       Pool P, int Dmax, double *out) {
-    const int b = blockIdx.x;
+                                                       Pool P, int Dmax, double *out, int first) {
+    const int b = first + blockIdx.x;  // population position; workspace blockIdx.x of P.scratch
     if (b >= *P.n) return;
     const int slot = P.order[b];
     __shared__ int ov[MQE_MAXL + 1], wv[MQE_MAXL + 1];
@@ -841,7 +857,7 @@ __global__ __launch_bounds__(MQE_NT) void overlap_kernel(int L, const int *ovd, 
     }
     __syncthreads();
     const T *phi = (const T *)P.phi + (size_t)slot * P.wflat;
-    T *E = (T *)(P.scratch + (size_t)b * P.scratch_bytes);
+    T *E = (T *)(P.scratch + (size_t)blockIdx.x * P.scratch_bytes);
     T *E2 = E + (size_t)Dmax * Dmax;
     T *Y = E2 + (size_t)Dmax * Dmax;
     set_one(E);
@@ -862,6 +878,132 @@ __global__ __launch_bounds__(MQE_NT) void overlap_kernel(int L, const int *ovd, 
         T *t = E; E = E2; E2 = t;
     }
     if (threadIdx.x == 0) out[b] = (double)E[0];
+}
+
+// ---------------------------------------------------------------------------
+// Recompression of walkers (MPSstate::CompressState, MPSstate.cpp:379-404):
+// a left-to-right and a right-to-left sweep of two-site SVDs
+// (TwoSiteObject::Compose / Decompose, TwoSiteObject.cpp:71-184), keeping
+// min(D, number of Schmidt values >= 1e-13) states per bond (:16, :147-157);
+// moving right the Schmidt values go to the right tensor, moving left to the
+// left one (:168, :177).  FP64 in global scratch, one workgroup per walker.
+// ---------------------------------------------------------------------------
+constexpr double SCHMIDT_CUT = 1e-13;
+
+// walker tensor element (row r, column c, physical s) of site k with dims (dl, dr)
+template <typename T>
+__device__ __forceinline__ T *wel(const Pool &P, T *phi, int k, int dl, int dr, int s, int r, int c) {
+    return P.padded ? phi + (size_t)(2 * k + s) * 1024 + r + 32 * c : phi + P.woff[k] + r + (size_t)dl * (c + (size_t)dr * s);
+}
+
+template <typename T>
+__device__ void compress_bond(const Pool &P, T *phi, int *wv, int k, int D, bool movingright, double *X, double *V,
+                              double *sig, int *ord, int *sh_flag) {
+    const int dl = wv[k], dm = wv[k + 1], dr = wv[k + 2];
+    const int m = 2 * dl, n = 2 * dr;
+    const bool tr = n > m;  // Jacobi on the columns of the orientation with fewer of them
+    const int rows = tr ? n : m, cols = tr ? m : n;
+    // Compose: theta(alpha + dl il, beta + dr ir) = sum_mid A_k[il](alpha, mid) A_{k+1}[ir](mid, beta)
+    for (int e = threadIdx.x; e < m * n; e += MQE_NT) {
+        const int row = e % m, col = e / m;
+        const int al = row % dl, il = row / dl, be = col % dr, ir = col / dr;
+        double s = 0.0;
+        for (int q = 0; q < dm; ++q)
+            s += (double)*wel(P, phi, k, dl, dm, il, al, q) * (double)*wel(P, phi, k + 1, dm, dr, ir, q, be);
+        if (tr) X[col + (size_t)rows * row] = s;
+        else X[row + (size_t)rows * col] = s;
+    }
+    __syncthreads();
+    jacobi_svd_columns(X, rows, cols, V, sig, sh_flag);
+    sort_descending(sig, cols, ord);
+    int keep = 0;
+    for (int j = 0; j < cols; ++j) keep += sig[ord[j]] >= SCHMIDT_CUT ? 1 : 0;
+    keep = max(1, min(keep, D));
+    // left singular vector j (length m) and right one (length n) of theta
+    auto u = [&](int j, int row) {
+        const int c = ord[j];
+        return tr ? V[row + (size_t)cols * c] : (sig[c] > 0.0 ? X[row + (size_t)rows * c] / sig[c] : 0.0);
+    };
+    auto vt = [&](int j, int col) {
+        const int c = ord[j];
+        return tr ? (sig[c] > 0.0 ? X[col + (size_t)rows * c] / sig[c] : 0.0) : V[col + (size_t)cols * c];
+    };
+    // Decompose: the new bond holds `keep` states (zero padding cleared on the tile layout)
+    if (P.padded) {
+        for (int e = threadIdx.x; e < 2 * 1024; e += MQE_NT) {
+            phi[(size_t)(2 * k) * 1024 + e] = (T)0;
+            phi[(size_t)(2 * k + 2) * 1024 + e] = (T)0;
+        }
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < m * keep; e += MQE_NT) {
+        const int row = e % m, j = e / m, al = row % dl, il = row / dl;
+        const double f = movingright ? 1.0 : sig[ord[j]];
+        *wel(P, phi, k, dl, keep, il, al, j) = (T)(u(j, row) * f);
+    }
+    for (int e = threadIdx.x; e < keep * n; e += MQE_NT) {
+        const int j = e % keep, col = e / keep, be = col % dr, ir = col / dr;
+        const double f = movingright ? sig[ord[j]] : 1.0;
+        *wel(P, phi, k + 1, keep, dr, ir, j, be) = (T)(vt(j, col) * f);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) wv[k + 1] = keep;
+    __syncthreads();
+}
+
+// Walkers first .. first+gridDim-1 (population order) to bond targets[b];
+// afterwards sites 1..L-1 are right-orthonormal and site 0 is scaled to unit
+// norm (the LeftNormalize of Walker::setOverlap, Walker.cpp:54-66, up to gauge).
+template <typename T>
+__global__ __launch_bounds__(MQE_NT) void compress_kernel(int L, Pool P, int first, const int *targets, char *scr,
+                                                          long scr_bytes, int Dmax) {
+    const int b = first + blockIdx.x;
+    if (b >= *P.n) return;
+    const int slot = P.order[b];
+    __shared__ int wv[MQE_MAXL + 1];
+    __shared__ int flag;
+    __shared__ double red[MQE_NT + 16 * MQE_WAVES];
+    for (int k = threadIdx.x; k <= L; k += MQE_NT) wv[k] = P.wvd[(size_t)slot * (L + 1) + k];
+    __syncthreads();
+    T *phi = (T *)P.phi + (size_t)slot * P.wflat;
+    const long n2 = 4L * Dmax * Dmax;
+    double *X = (double *)(scr + (size_t)blockIdx.x * scr_bytes), *V = X + n2, *sig = V + n2;
+    int *ord = (int *)(sig + 2 * Dmax);
+    const int D = targets[b];
+    if (D < 1) return;  // walker left as it is
+    for (int k = 0; k < L - 1; ++k) compress_bond(P, phi, wv, k, D, true, X, V, sig, ord, &flag);
+    for (int k = L - 2; k >= 0; --k) compress_bond(P, phi, wv, k, D, false, X, V, sig, ord, &flag);
+    const int w1 = wv[1];
+    double s2 = 0.0;
+    for (int e = threadIdx.x; e < 2 * w1; e += MQE_NT) {
+        const double v = (double)*wel(P, phi, 0, 1, w1, e / w1, 0, e % w1);
+        s2 += v * v;
+    }
+    const double inv = 1.0 / sqrt(block_sum(s2, red));
+    for (int e = threadIdx.x; e < 2 * w1; e += MQE_NT) {
+        T *p = wel(P, phi, 0, 1, w1, e / w1, 0, e % w1);
+        *p = (T)((double)*p * inv);
+    }
+    for (int k = threadIdx.x; k <= L; k += MQE_NT) P.wvd[(size_t)slot * (L + 1) + k] = wv[k];
+}
+
+// Walker::setOverlap's sign convention (Walker.cpp:59-62): a negative overlap
+// with the trial flips site 0; the stored overlap is |<trial|walker>|.
+template <typename T>
+__global__ void overlap_sign_kernel(int L, Pool P, const double *ov, const int *targets) {
+    const int b = blockIdx.x;
+    if (b >= *P.n || targets[b] < 1) return;
+    const int slot = P.order[b];
+    const double o = ov[b];
+    if (o < 0.0) {
+        T *phi = (T *)P.phi + (size_t)slot * P.wflat;
+        const int w1 = P.wvd[(size_t)slot * (L + 1) + 1];
+        for (int e = threadIdx.x; e < 2 * w1; e += blockDim.x) {
+            T *p = wel(P, phi, 0, 1, w1, e / w1, 0, e % w1);
+            *p = -*p;
+        }
+    }
+    if (threadIdx.x == 0) P.O[slot] = fabs(o);
 }
 
 }  // namespace mqe
@@ -935,33 +1077,48 @@ __global__ void advance_ctl_kernel(Ctl *ctl) {
     }
 }
 
-// Population positions in launch order: the walkers that re-orthonormalise in
-// this propagation (several times the work of the others) first, so that they
-// start in the first round of resident workgroups instead of extending the
-// last one; each group keeps population order.
-__global__ __launch_bounds__(RED_NT) void launch_list_kernel(const int *n_ptr, const int *order, const Ctl *ctl,
-                                                             int *list) {
+// Population positions in launch order, most work first: the walkers that
+// re-orthonormalise in this propagation (several times the work of a light
+// step), then by bond bucket from the largest bonds down, each class in
+// population order.  Heavy walkers start in the first round of resident
+// workgroups instead of extending the last one.  key: scratch, one int per position.
+__global__ __launch_bounds__(RED_NT) void launch_list_kernel(const int *n_ptr, const int *order, const int *wvd,
+                                                             int L, const Ctl *ctl, int *key, int *list) {
     __shared__ int shi[RED_NT / 64];
-    __shared__ int carry_h, carry_l;
+    __shared__ int cnt[16], base[16];
+    __shared__ int carry;
     const int n = *n_ptr;
     const Ctl c = *ctl;
-    int nh = 0;
-    for (int b = threadIdx.x; b < n; b += RED_NT) nh += canon_slot(c, order[b]) ? 1 : 0;
-    int ex;
-    const int nheavy = rscan_excl(nh, &ex, shi);
-    if (threadIdx.x == 0) { carry_h = 0; carry_l = 0; }
+    if (threadIdx.x < 16) cnt[threadIdx.x] = 0;
     __syncthreads();
-    for (int base = 0; base < n; base += RED_NT) {
-        const int b = base + threadIdx.x;
-        const int h = (b < n && canon_slot(c, order[b])) ? 1 : 0;
-        const int toth = rscan_excl(h, &ex, shi);
-        if (b < n) list[h ? carry_h + ex : nheavy + carry_l + (threadIdx.x - ex)] = b;
+    for (int b = threadIdx.x; b < n; b += RED_NT) {
+        const int slot = order[b];
+        int mb = 1;
+        for (int k = 0; k <= L; ++k) mb = max(mb, wvd[(size_t)slot * (L + 1) + k]);
+        const int kk = (canon_slot(c, slot) ? 8 : 0) + bond_bucket(mb);
+        key[b] = kk;
+        atomicAdd(&cnt[kk], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int kk = 15; kk >= 0; --kk) { base[kk] = run; run += cnt[kk]; }
+    }
+    __syncthreads();
+    for (int kk = 15; kk >= 0; --kk) {
+        if (cnt[kk] == 0) continue;  // block-uniform
+        if (threadIdx.x == 0) carry = base[kk];
         __syncthreads();
-        if (threadIdx.x == 0) {
-            carry_h += toth;
-            carry_l += min(RED_NT, n - base) - toth;
+        for (int b0 = 0; b0 < n; b0 += RED_NT) {
+            const int b = b0 + threadIdx.x;
+            const int f = (b < n && key[b] == kk) ? 1 : 0;
+            int ex;
+            const int tot = rscan_excl(f, &ex, shi);
+            if (f) list[carry + ex] = b;
+            __syncthreads();
+            if (threadIdx.x == 0) carry += tot;
+            __syncthreads();
         }
-        __syncthreads();
     }
 }
 
@@ -1273,7 +1430,7 @@ struct mqe_engine {
     double *d_tmpout = nullptr, *d_nw = nullptr;
     Ctl *d_ctl = nullptr;                      // step control block (device)
     Ctl ctl_host{};                            // its last value set from the host
-    int *d_list = nullptr;                     // launch order of the walker kernel
+    int *d_list = nullptr, *d_key = nullptr;   // launch order of the walker kernel and its class keys
     long prop_draws = 0, pop_draws = 0;        // step words of mqe_propagate / mqe_population_control without host draws
     int n_host = 0;
     bool loaded = false, prepared = false;
@@ -1383,8 +1540,8 @@ static int launch_walker_any(mqe_engine *e, const double *u, int flags) {
     const bool prop = (flags & F_PROP) != 0;
     if (prop) {
         ++e->prop_count;
-        hipLaunchKernelGGL(launch_list_kernel, dim3(1), dim3(RED_NT), 0, e->stream, e->d_n, e->d_order, e->d_ctl,
-                           e->d_list);
+        hipLaunchKernelGGL(launch_list_kernel, dim3(1), dim3(RED_NT), 0, e->stream, e->d_n, e->d_order, e->d_wvd, e->L,
+                           e->d_ctl, e->d_key, e->d_list);
         HIPCHK(hipGetLastError());
         P.list = e->d_list;
     }
@@ -1415,6 +1572,22 @@ static int launch_walker_any(mqe_engine *e, const double *u, int flags) {
     } else if (e->fp32) launch_walker<float>(e, P, flags);
     else launch_walker<double>(e, P, flags);
     HIPCHK(hipGetLastError());
+    return MQE_OK;
+}
+
+// the population size after the work queued so far
+static int refresh_n(mqe_engine *e) {
+    HIPCHK(hipStreamSynchronize(e->stream));
+    HIPCHK(hipMemcpy(&e->n_host, e->d_n, sizeof(int), hipMemcpyDeviceToHost));
+    return MQE_OK;
+}
+
+// the population of the last propagation (recorded by its weight sum)
+static int refresh_last_walkers(mqe_engine *e) {
+    double n = 0.0;
+    HIPCHK(hipStreamSynchronize(e->stream));
+    HIPCHK(hipMemcpy(&n, e->d_sums + 5, sizeof(double), hipMemcpyDeviceToHost));
+    e->last_walkers = (int)n;
     return MQE_OK;
 }
 
@@ -1463,6 +1636,103 @@ static int enqueue_popcontrol(mqe_engine *e, double scaling, const double *du, i
     HIPCHK(hipGetLastError());
     return MQE_OK;
 }
+
+static bool tune_cache_read(const mqe_engine *e, int *tiles);
+
+// ---- overlaps with another MPS in a caller-provided device workspace ----------
+// Layout: the other MPS (engine precision), its bond dims and site offsets,
+// then one transfer-matrix workspace (E, E2: Dmax^2, Y: Dmax x 2 Dmax) per
+// walker of a chunk; walkers run in chunks of as many workspaces as fit.
+namespace mqe {
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct OverlapPlan {
+    int Dmax;
+    long tot;               // elements of the other MPS
+    size_t off_vd, off_off, off_ws, per_walker;
+};
+
+static int plan_overlaps(mqe_engine *e, const mqe_mps *other, OverlapPlan *pl) {
+    if (!e || !other || !other->vd || !other->data || other->L != e->L || other->d != 2) {
+        set_error("mqe_overlaps: bad arguments");
+        return MQE_EINVAL;
+    }
+    const int L = e->L;
+    if (other->vd[0] != 1 || other->vd[L] != 1) { set_error("mqe_overlaps: boundary bonds must be 1"); return MQE_EINVAL; }
+    pl->Dmax = e->Dw;
+    pl->tot = 0;
+    for (int k = 0; k < L; ++k) {
+        if (other->vd[k] < 1 || other->vd[k] > 4096) { set_error("mqe_overlaps: bad bond dimension"); return MQE_EINVAL; }
+        pl->tot += (long)other->vd[k] * other->vd[k + 1] * 2;
+        pl->Dmax = std::max(pl->Dmax, other->vd[k]);
+    }
+    const size_t tsz = e->fp32 ? sizeof(float) : sizeof(double);
+    pl->off_vd = align256(tsz * (size_t)pl->tot);
+    pl->off_off = pl->off_vd + align256(sizeof(int) * (size_t)(L + 1));
+    pl->off_ws = pl->off_off + align256(sizeof(long) * (size_t)L);
+    pl->per_walker = align256(tsz * 4 * (size_t)pl->Dmax * pl->Dmax);
+    return MQE_OK;
+}
+
+int overlaps_workspace(mqe_engine *e, const mqe_mps *other, long *bytes_all, long *bytes_min) {
+    OverlapPlan pl;
+    ECHK(plan_overlaps(e, other, &pl));
+    ECHK(refresh_n(e));
+    if (bytes_all) *bytes_all = (long)(pl.off_ws + pl.per_walker * (size_t)std::max(e->n_host, 1));
+    if (bytes_min) *bytes_min = (long)(pl.off_ws + pl.per_walker);
+    return MQE_OK;
+}
+
+int overlaps_in_workspace(mqe_engine *e, const mqe_mps *other, double *out, char *ws, long ws_bytes) {
+    OverlapPlan pl;
+    ECHK(plan_overlaps(e, other, &pl));
+    if (!out || !ws) { set_error("mqe_overlaps: bad arguments"); return MQE_EINVAL; }
+    ECHK(refresh_n(e));
+    const int L = e->L, n = e->n_host;
+    if ((size_t)ws_bytes < pl.off_ws + pl.per_walker) {
+        set_error("mqe_overlaps: the workspace cannot hold the other MPS and one walker's transfer matrices");
+        return MQE_ECAPACITY;
+    }
+    const int chunk = (int)std::min<size_t>((size_t)std::max(n, 1), ((size_t)ws_bytes - pl.off_ws) / pl.per_walker);
+    // stage the other MPS in the engine's precision, its dims and offsets
+    // (fixed-size host buffers: this path allocates nothing on host or device)
+    long off[MQE_MAXL];
+    long t = 0;
+    for (int k = 0; k < L; ++k) {
+        off[k] = t;
+        t += (long)other->vd[k] * other->vd[k + 1] * 2;
+    }
+    if (e->fp32) {
+        float tmp[4096];
+        for (long i0 = 0; i0 < pl.tot; i0 += 4096) {
+            const long m = std::min(4096L, pl.tot - i0);
+            for (long i = 0; i < m; ++i) tmp[i] = (float)other->data[i0 + i];
+            HIPCHK(hipMemcpy(ws + sizeof(float) * i0, tmp, sizeof(float) * m, hipMemcpyHostToDevice));
+        }
+    } else {
+        HIPCHK(hipMemcpy(ws, other->data, sizeof(double) * pl.tot, hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMemcpy(ws + pl.off_vd, other->vd, sizeof(int) * (L + 1), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ws + pl.off_off, off, sizeof(long) * L, hipMemcpyHostToDevice));
+    Pool P = make_pool(e, nullptr);
+    P.scratch = ws + pl.off_ws;
+    P.scratch_bytes = (long)pl.per_walker;
+    const int *ovd = (const int *)(ws + pl.off_vd);
+    const long *ooff = (const long *)(ws + pl.off_off);
+    for (int f = 0; f < n; f += chunk) {
+        if (e->fp32)
+            hipLaunchKernelGGL(overlap_kernel<float>, dim3(chunk), dim3(MQE_NT), 0, e->stream, L, ovd, ooff,
+                               (const float *)ws, P, pl.Dmax, e->d_tmpout, f);
+        else
+            hipLaunchKernelGGL(overlap_kernel<double>, dim3(chunk), dim3(MQE_NT), 0, e->stream, L, ovd, ooff,
+                               (const double *)ws, P, pl.Dmax, e->d_tmpout, f);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    HIPCHK(hipMemcpy(out, e->d_tmpout, sizeof(double) * n, hipMemcpyDeviceToHost));
+    return MQE_OK;
+}
+}  // namespace mqe
 
 extern "C" {
 
@@ -1672,6 +1942,7 @@ int mqe_create(const mqe_params *p, const mqe_mps *trial, int n_couplings, const
     CK(hipMalloc((void **)&e->d_tmpout, sizeof(double) * cap));
     CK(hipMalloc((void **)&e->d_nw, sizeof(double) * cap));
     CK(hipMalloc((void **)&e->d_list, sizeof(int) * cap));
+    CK(hipMalloc((void **)&e->d_key, sizeof(int) * cap));
     CK(hipMalloc((void **)&e->d_ctl, sizeof(Ctl)));
 
     Sys &S = e->sys;
@@ -1707,6 +1978,11 @@ int mqe_create(const mqe_params *p, const mqe_mps *trial, int n_couplings, const
     S.h = p->field;
     S.ssp[0] = std::exp(0.5 * p->field * p->dtau * (-0.5));  // TrotterHeisenberg.cpp:86
     S.ssp[1] = std::exp(0.5 * p->field * p->dtau * (0.5));
+    for (int q = 0; q < 8; ++q) S.tiles[q] = q == 0 ? TILE_16 : TILE_32;  // default table
+    {
+        int t[8];
+        if (tune_cache_read(e, t)) for (int q = 0; q < 8; ++q) S.tiles[q] = t[q];  // tuned by mqe_autotune
+    }
     if (push_ctl(e) != MQE_OK) return fail(MQE_EDEVICE);
     CK(hipStreamSynchronize(e->stream));
     *out = e;
@@ -1724,7 +2000,7 @@ void mqe_destroy(mqe_engine *e) {
                     e->d_n, e->d_idx, e->d_u, e->d_scratch, e->d_status, e->d_sums, e->d_trace, e->d_ncopy,
                     e->d_newpos, e->d_used, e->d_free, e->d_csrc, e->d_cdst, e->d_nextra, e->d_nnew, e->d_nused,
                     e->d_tmpout, e->d_bond, e->d_feat, e->d_pred, e->d_sX, e->d_sY, e->d_nsamp, e->d_plab,
-                    e->d_nw, e->d_list, e->d_ctl};
+                    e->d_nw, e->d_list, e->d_key, e->d_ctl};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     for (hipEvent_t ev : e->tev) (void)hipEventDestroy(ev);
@@ -2015,6 +2291,192 @@ int mqe_read_trace(mqe_engine *e, int first_step, int n_steps, double *out) {
 
 int mqe_kernel_path(mqe_engine *e) { return (e && e->tile32) ? 1 : 0; }
 
+}  // extern "C"
+
+// ---- tile-shape autotuner -----------------------------------------------------
+// Key of the tuned table: the product shapes depend on the trial and walker
+// bonds, the chain length, the couplings and the precision.
+static std::string tune_key(const mqe_engine *e) {
+    char b[160];
+    snprintf(b, sizeof(b), "L%d_nC%d_Dt%d_Dw%d_d%d_%s", e->L, e->nC, e->Dt, e->Dw, e->p.d, e->fp32 ? "fp32" : "fp64");
+    return b;
+}
+
+static std::string tune_cache_path() {
+    const char *p = getenv("MQE_TUNE_CACHE");
+    if (p && p[0]) return p;
+    const char *h = getenv("HOME");
+    return std::string(h && h[0] ? h : "/tmp") + "/.cache/mqe_tiles.txt";
+}
+
+// the cached table for this engine's key, if any (lines "key t0 ... t7")
+static bool tune_cache_read(const mqe_engine *e, int *tiles) {
+    FILE *f = std::fopen(tune_cache_path().c_str(), "r");
+    if (!f) return false;
+    const std::string key = tune_key(e);
+    char k[160];
+    int t[8];
+    bool found = false;
+    while (std::fscanf(f, "%159s %d %d %d %d %d %d %d %d", k, &t[0], &t[1], &t[2], &t[3], &t[4], &t[5], &t[6],
+                       &t[7]) == 9)
+        if (key == k) {
+            found = true;
+            for (int q = 0; q < 8; ++q) tiles[q] = t[q];
+        }
+    std::fclose(f);
+    for (int q = 0; found && q < 8; ++q)
+        if (tiles[q] != TILE_16 && tiles[q] != TILE_32 && tiles[q] != TILE_WIDE) found = false;
+    return found;
+}
+
+static void tune_cache_write(const mqe_engine *e, const int *tiles) {
+    const std::string path = tune_cache_path();
+    const size_t sl = path.rfind('/');
+    if (sl != std::string::npos && sl > 0) (void)mkdir(path.substr(0, sl).c_str(), 0755);  // EEXIST is fine
+    FILE *f = std::fopen(path.c_str(), "a");
+    if (!f) return;  // a read-only home: the table stays in this engine only
+    std::fprintf(f, "%s", tune_key(e).c_str());
+    for (int q = 0; q < 8; ++q) std::fprintf(f, " %d", tiles[q]);
+    std::fprintf(f, "\n");
+    std::fclose(f);
+}
+
+static void set_tiles(mqe_engine *e, const int *tiles) {
+    for (int q = 0; q < 8; ++q) e->sys.tiles[q] = tiles[q];
+    if (e->graph_exec) {  // captured launches carry the old table
+        (void)hipGraphExecDestroy(e->graph_exec);
+        e->graph_exec = nullptr;
+        e->graph_steps = 0;
+    }
+}
+
+extern "C" {
+
+int mqe_get_tiles(mqe_engine *e, int *tiles) {
+    if (!e || !tiles) { set_error("mqe_get_tiles: bad arguments"); return MQE_EINVAL; }
+    for (int q = 0; q < 8; ++q) tiles[q] = e->sys.tiles[q];
+    return MQE_OK;
+}
+
+int mqe_set_tiles(mqe_engine *e, const int *tiles) {
+    if (!e || !tiles) { set_error("mqe_set_tiles: bad arguments"); return MQE_EINVAL; }
+    for (int q = 0; q < 8; ++q)
+        if (tiles[q] != TILE_16 && tiles[q] != TILE_32 && tiles[q] != TILE_WIDE) {
+            set_error("mqe_set_tiles: tile shapes are 16, 32 or 64");
+            return MQE_EINVAL;
+        }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    set_tiles(e, tiles);
+    return MQE_OK;
+}
+
+int mqe_autotune(mqe_engine *e, int reps, int *tiles_out, double *ms_default, double *ms_tuned) {
+    if (!e || reps < 1) { set_error("mqe_autotune: bad arguments"); return MQE_EINVAL; }
+    if (!e->prepared) { set_error("mqe_autotune: call mqe_prepare first"); return MQE_ESTATE; }
+    int best[8];
+    for (int q = 0; q < 8; ++q) best[q] = e->sys.tiles[q];
+    if (!e->fp32 || e->tile32) {  // only the general path's FP32 products have tile shapes
+        if (tiles_out) for (int q = 0; q < 8; ++q) tiles_out[q] = best[q];
+        if (ms_default) *ms_default = 0.0;
+        if (ms_tuned) *ms_tuned = 0.0;
+        return MQE_OK;
+    }
+    ECHK(refresh_n(e));
+    const int cap = e->p.capacity, L = e->L;
+    // buckets present in the population
+    std::vector<int> order(cap), vd((size_t)cap * (L + 1));
+    HIPCHK(hipMemcpy(order.data(), e->d_order, sizeof(int) * cap, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(vd.data(), e->d_wvd, sizeof(int) * vd.size(), hipMemcpyDeviceToHost));
+    bool present[8] = {false};
+    for (int b = 0; b < e->n_host; ++b) {
+        int mb = 1;
+        for (int k = 0; k <= L; ++k) mb = std::max(mb, vd[(size_t)order[b] * (L + 1) + k]);
+        present[bond_bucket(mb)] = true;
+    }
+    // snapshot of everything a propagation writes
+    const size_t tsz = sizeof(float);
+    struct Part { void *dev; size_t bytes; void *save; };
+    std::vector<Part> parts = {{e->d_phi, tsz * e->wflat * cap, nullptr},  {e->d_Renv, tsz * e->eflat * cap, nullptr},
+                               {e->d_Lenv, tsz * e->eflat * cap, nullptr}, {e->d_wvd, sizeof(int) * (L + 1) * cap, nullptr},
+                               {e->d_w, sizeof(double) * cap, nullptr},     {e->d_O, sizeof(double) * cap, nullptr},
+                               {e->d_eh, sizeof(double) * cap, nullptr},    {e->d_EL, sizeof(double) * cap, nullptr},
+                               {e->d_order, sizeof(int) * cap, nullptr},    {e->d_n, sizeof(int), nullptr},
+                               {e->d_status, sizeof(int), nullptr}};
+    int rc = MQE_OK;
+    for (auto &pt : parts)
+        if (rc == MQE_OK) {
+            if (hipMalloc(&pt.save, pt.bytes) != hipSuccess ||
+                hipMemcpyAsync(pt.save, pt.dev, pt.bytes, hipMemcpyDeviceToDevice, e->stream) != hipSuccess)
+                rc = MQE_EDEVICE;
+        }
+    const Ctl ctl0 = e->ctl_host;
+    const long pc0 = e->prop_count;
+    auto restore = [&]() -> int {
+        for (auto &pt : parts)
+            if (pt.save && hipMemcpyAsync(pt.dev, pt.save, pt.bytes, hipMemcpyDeviceToDevice, e->stream) != hipSuccess)
+                return MQE_EDEVICE;
+        e->ctl_host = ctl0;
+        e->prop_count = pc0;
+        return MQE_OK;
+    };
+    // device time of `reps` propagations with the table t (state restored afterwards)
+    auto trial = [&](const int *t, double *ms) -> int {
+        set_tiles(e, t);
+        float tot = 0.f;
+        for (int r = 0; r < reps; ++r) {
+            e->ctl_host.step = ctl0.step + 1 + r;
+            e->ctl_host.pc = e->prop_count;
+            int q = push_ctl(e);
+            if (q != MQE_OK) return q;
+            if (hipEventRecord(e->ev0, e->stream) != hipSuccess) return MQE_EDEVICE;
+            q = launch_walker_any(e, nullptr, F_PROP);
+            if (q != MQE_OK) return q;
+            if (hipEventRecord(e->ev1, e->stream) != hipSuccess || hipEventSynchronize(e->ev1) != hipSuccess)
+                return MQE_EDEVICE;
+            float x = 0.f;
+            (void)hipEventElapsedTime(&x, e->ev0, e->ev1);
+            tot += x;
+        }
+        *ms = tot / reps;
+        return restore();
+    };
+    double t_default = 0.0, t_best = 0.0;
+    if (rc == MQE_OK) rc = trial(best, &t_default);  // also warms the code up
+    if (rc == MQE_OK) rc = trial(best, &t_default);
+    t_best = t_default;
+    // coordinate descent over the buckets present, largest bonds first
+    const int cands[3] = {TILE_16, TILE_32, TILE_WIDE};
+    for (int bk = 7; rc == MQE_OK && bk >= 0; --bk) {
+        if (!present[bk]) continue;
+        for (int c : cands) {
+            if (c == best[bk] || rc != MQE_OK) continue;
+            int t[8];
+            for (int q = 0; q < 8; ++q) t[q] = best[q];
+            t[bk] = c;
+            double ms = 0.0;
+            rc = trial(t, &ms);
+            if (rc == MQE_OK && ms < 0.98 * t_best) {  // keep a change only for a clear gain
+                t_best = ms;
+                best[bk] = c;
+            }
+        }
+    }
+    if (rc == MQE_OK) (void)restore();
+    for (auto &pt : parts)
+        if (pt.save) (void)hipFree(pt.save);
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (rc != MQE_OK) {
+        set_error("mqe_autotune: device error while timing a tile table");
+        return rc;
+    }
+    set_tiles(e, best);
+    tune_cache_write(e, best);
+    if (tiles_out) for (int q = 0; q < 8; ++q) tiles_out[q] = best[q];
+    if (ms_default) *ms_default = t_default;
+    if (ms_tuned) *ms_tuned = t_best;
+    return MQE_OK;
+}
+
 int mqe_set_canonical_interval(mqe_engine *e, int steps) {
     if (!e || steps < 1) { set_error("mqe_set_canonical_interval: bad arguments"); return MQE_EINVAL; }
     e->canon_every = steps;
@@ -2042,22 +2504,6 @@ int mqe_num_walkers(mqe_engine *e, int *n) {
 int mqe_num_couplings(mqe_engine *e, int *n) {
     if (!e || !n) { set_error("mqe_num_couplings: bad arguments"); return MQE_EINVAL; }
     *n = e->nC;
-    return MQE_OK;
-}
-
-// the population size after the work queued so far
-static int refresh_n(mqe_engine *e) {
-    HIPCHK(hipStreamSynchronize(e->stream));
-    HIPCHK(hipMemcpy(&e->n_host, e->d_n, sizeof(int), hipMemcpyDeviceToHost));
-    return MQE_OK;
-}
-
-// the population of the last propagation (recorded by its weight sum)
-static int refresh_last_walkers(mqe_engine *e) {
-    double n = 0.0;
-    HIPCHK(hipStreamSynchronize(e->stream));
-    HIPCHK(hipMemcpy(&n, e->d_sums + 5, sizeof(double), hipMemcpyDeviceToHost));
-    e->last_walkers = (int)n;
     return MQE_OK;
 }
 
@@ -2120,48 +2566,62 @@ int mqe_read_walker(mqe_engine *e, int i, int *vd, double *data, long *size) {
 }
 
 int mqe_overlaps(mqe_engine *e, const mqe_mps *other, double *out) {
-    if (!e || !other || !out || other->L != e->L || other->d != 2) { set_error("mqe_overlaps: bad arguments"); return MQE_EINVAL; }
-    const int L = e->L;
-    if (other->vd[0] != 1 || other->vd[L] != 1) { set_error("mqe_overlaps: boundary bonds must be 1"); return MQE_EINVAL; }
-    ECHK(refresh_n(e));
-    int Dmax = e->Dw;
-    std::vector<long> off(L);
-    long tot = 0;
-    for (int k = 0; k < L; ++k) {
-        off[k] = tot;
-        tot += (long)other->vd[k] * other->vd[k + 1] * 2;
-        Dmax = std::max(Dmax, other->vd[k]);
-    }
-    const size_t tsz = e->fp32 ? sizeof(float) : sizeof(double);
-    // scratch: E, E2 (Dmax^2 each) and Y (Dmax x 2 Dmax)
-    const size_t need = tsz * 4 * (size_t)Dmax * Dmax;
+    long need = 0;
+    ECHK(mqe::overlaps_workspace(e, other, &need, nullptr));
     char *scr = nullptr;
-    HIPCHK(hipMalloc((void **)&scr, need * std::max(e->n_host, 1)));
-    void *d_other = nullptr;
-    int r = e->fp32 ? upload_mps_typed<float>(other->data, tot, &d_other) : upload_mps_typed<double>(other->data, tot, &d_other);
-    if (r) { (void)hipFree(scr); return r; }
-    int *d_ovd = nullptr;
-    long *d_ooff = nullptr;
-    HIPCHK(hipMalloc((void **)&d_ovd, sizeof(int) * (L + 1)));
-    HIPCHK(hipMalloc((void **)&d_ooff, sizeof(long) * L));
-    HIPCHK(hipMemcpy(d_ovd, other->vd, sizeof(int) * (L + 1), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(d_ooff, off.data(), sizeof(long) * L, hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc((void **)&scr, (size_t)need));
+    const int r = mqe::overlaps_in_workspace(e, other, out, scr, need);
+    (void)hipFree(scr);
+    return r;
+}
+
+int mqe_compress_walkers(mqe_engine *e, const int *targets) {
+    if (!e || !targets) { set_error("mqe_compress_walkers: bad arguments"); return MQE_EINVAL; }
+    if (!e->loaded) { set_error("mqe_compress_walkers: no walkers loaded"); return MQE_ESTATE; }
+    ECHK(refresh_n(e));
+    const int n = e->n_host, L = e->L;
+    const int Dmax = e->Dw;
+    // per walker: X and V ((2 Dmax)^2 doubles each), Schmidt values and their order
+    const long scr_bytes = ((8L * (2 * 4L * Dmax * Dmax + 2 * Dmax) + 4L * 2 * Dmax) + 255) & ~255L;
+    const int chunk = std::max(1, std::min(n, (int)std::min(1024L, (1L << 30) / scr_bytes)));
+    char *scr = nullptr;
+    int *dt = nullptr;
+    HIPCHK(hipMalloc((void **)&scr, (size_t)scr_bytes * chunk));
+    HIPCHK(hipMalloc((void **)&dt, sizeof(int) * n));
+    HIPCHK(hipMemcpy(dt, targets, sizeof(int) * n, hipMemcpyHostToDevice));
     Pool P = make_pool(e, nullptr);
+    for (int f = 0; f < n; f += chunk) {
+        if (e->fp32)
+            hipLaunchKernelGGL(compress_kernel<float>, dim3(chunk), dim3(MQE_NT), 0, e->stream, L, P, f, dt, scr,
+                               scr_bytes, Dmax);
+        else
+            hipLaunchKernelGGL(compress_kernel<double>, dim3(chunk), dim3(MQE_NT), 0, e->stream, L, P, f, dt, scr,
+                               scr_bytes, Dmax);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    (void)hipFree(scr);
+    // overlaps with the trial (transfer matrices), then the sign convention
+    const int Do = std::max(e->Dw, e->Dt);
+    const size_t tsz = e->fp32 ? sizeof(float) : sizeof(double);
+    const size_t need = tsz * 4 * (size_t)Do * Do;
+    HIPCHK(hipMalloc((void **)&scr, need * std::max(n, 1)));
     P.scratch = scr;
     P.scratch_bytes = (long)need;
-    if (e->fp32)
-        hipLaunchKernelGGL(overlap_kernel<float>, dim3(e->n_host), dim3(MQE_NT), 0, e->stream, L, d_ovd, d_ooff,
-                           (const float *)d_other, P, Dmax, e->d_tmpout);
-    else
-        hipLaunchKernelGGL(overlap_kernel<double>, dim3(e->n_host), dim3(MQE_NT), 0, e->stream, L, d_ovd, d_ooff,
-                           (const double *)d_other, P, Dmax, e->d_tmpout);
+    if (e->fp32) {
+        hipLaunchKernelGGL(overlap_kernel<float>, dim3(n), dim3(MQE_NT), 0, e->stream, L, e->d_tvd, e->d_toff,
+                           (const float *)e->d_trial, P, Do, e->d_tmpout, 0);
+        hipLaunchKernelGGL(overlap_sign_kernel<float>, dim3(n), dim3(64), 0, e->stream, L, P, e->d_tmpout, dt);
+    } else {
+        hipLaunchKernelGGL(overlap_kernel<double>, dim3(n), dim3(MQE_NT), 0, e->stream, L, e->d_tvd, e->d_toff,
+                           (const double *)e->d_trial, P, Do, e->d_tmpout, 0);
+        hipLaunchKernelGGL(overlap_sign_kernel<double>, dim3(n), dim3(64), 0, e->stream, L, P, e->d_tmpout, dt);
+    }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(e->stream));
-    HIPCHK(hipMemcpy(out, e->d_tmpout, sizeof(double) * e->n_host, hipMemcpyDeviceToHost));
     (void)hipFree(scr);
-    (void)hipFree(d_other);
-    (void)hipFree(d_ovd);
-    (void)hipFree(d_ooff);
+    (void)hipFree(dt);
+    e->prepared = false;
     return MQE_OK;
 }
 

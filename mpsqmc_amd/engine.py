@@ -87,6 +87,33 @@ class Engine:
         """n copies of one MPS (SetupWalkers with copyTrial = true)."""
         self.load_walkers([vd] * n, [data] * n, np.ones(n), np.full(n, O), np.zeros(n))
 
+    def compress_walkers(self, targets):
+        """CompressState(targets[i]) + setOverlap of every walker on the device
+        (targets[i] < 1: unchanged); call prepare() afterwards."""
+        t = np.ascontiguousarray(targets, dtype=np.int32)
+        assert t.size == self.num_walkers()
+        check(lib().mqe_compress_walkers(self.h, _iptr(t)))
+
+    # -- tile shapes of the general path ------------------------------------
+    @property
+    def tiles(self):
+        """FP32 product tile shape (16, 32 or 64) per walker bond bucket (8 entries)."""
+        t = np.zeros(8, dtype=np.int32)
+        check(lib().mqe_get_tiles(self.h, _iptr(t)))
+        return t
+
+    def set_tiles(self, tiles):
+        t = np.ascontiguousarray(tiles, dtype=np.int32)
+        assert t.size == 8
+        check(lib().mqe_set_tiles(self.h, _iptr(t)))
+
+    def autotune(self, reps=2):
+        """Tune the tile table on the current population; returns (tiles, ms_default, ms_tuned)."""
+        t = np.zeros(8, dtype=np.int32)
+        a, b = ctypes.c_double(), ctypes.c_double()
+        check(lib().mqe_autotune(self.h, int(reps), _iptr(t), ctypes.byref(a), ctypes.byref(b)))
+        return t, a.value, b.value
+
     # -- the step -----------------------------------------------------------
     def prepare(self):
         E = ctypes.c_double()

@@ -201,13 +201,16 @@ static double inner(const omps *A, const omps *B) {
         for (int s = 0; s < d; s++) {
             const double *As = A->t[k] + (long)s * la * ra;
             const double *Bs = B->t[k] + (long)s * lb * rb;
-            /* W (lb x ra) = E (lb x la) * As (la x ra) */
-            for (int c = 0; c < ra; c++)
-                for (int r = 0; r < lb; r++) {
-                    double acc = 0.0;
-                    for (int q = 0; q < la; q++) acc += E[r + lb * q] * As[q + la * c];
-                    W[r + lb * c] = acc;
+            /* W (lb x ra) = E (lb x la) * As (la x ra), columns of E streamed */
+            for (int c = 0; c < ra; c++) {
+                double *w = W + (long)lb * c;
+                for (int r = 0; r < lb; r++) w[r] = 0.0;
+                for (int q = 0; q < la; q++) {
+                    const double a = As[q + la * c];
+                    const double *e = E + (long)lb * q;
+                    for (int r = 0; r < lb; r++) w[r] += e[r] * a;
                 }
+            }
             /* E2 (rb x ra) += Bs^T (rb x lb) * W (lb x ra) */
             for (int c = 0; c < ra; c++)
                 for (int r = 0; r < rb; r++) {
@@ -953,6 +956,7 @@ typedef struct {
     int *idx;               /* sampled grid point per (walker, coupling), last propagate */
     double *EL;             /* last local energies */
     int nthreads;
+    int inner_threads;      /* threads over the 16 combos of one walker (fewer walkers than threads) */
     int emode;              /* 0: <H trial|phi> with uncompressed H|trial>; 1: per-coupling operator insertion;
                                2: <H trial|phi> with H|trial> from the minimal-channel MPO */
     omps hmin;
@@ -1042,19 +1046,15 @@ ORC_API orc_pop *orc_pop_create(int L, int d, const int *tvd, const double *tdat
     orc_pop *p = (orc_pop *)calloc(1, sizeof(orc_pop));
     p->L = L; p->d = d; p->n2 = d * d; p->h = h; p->dtau = dtau;
     p->nthreads = 1;
+    p->inner_threads = 1;
     mps_from_flat(&p->trial, L, d, tvd, tdata);
     (void)left_normalize(&p->trial);
     p->DT = p->trial.D;
-    /* H|trial> (MPSQMC2.cpp:255-257); the lossless compression drops only
-       Schmidt values < 1e-13 and is skipped here. */
-    ompo M; M.L = L; M.d = d; M.J = J; M.h = h;
-    M.dims = (int *)malloc(sizeof(int) * (L + 1));
-    mpo_dims(L, M.dims);
-    apply_mpo(&M, &p->trial, &p->htrial);
-    free(M.dims);
+    /* H|trial> (MPSQMC2.cpp:255-257) is built on first use (hamiltonian_of):
+       at L = 36, D = 128 the uncompressed state holds tens of GB, and the
+       operator-insertion energy (mode 1) does not need it. */
     p->Jfull = (double *)malloc(sizeof(double) * (L * (L - 1) / 2 > 0 ? L * (L - 1) / 2 : 1));
     memcpy(p->Jfull, J, sizeof(double) * (L * (L - 1) / 2));
-    apply_mpo_min(L, d, J, h, &p->trial, &p->hmin);
     int nC = 0;
     for (int i = 0; i < L - 1; i++)
         for (int j = i + 1; j < L; j++) if (J[(j * (j - 1)) / 2 + i] != 0.0) nC++;
@@ -1187,7 +1187,10 @@ ORC_API long orc_pop_get_trial(const orc_pop *p, int *vd, double *data) {
     return flat_size(p->L, p->d, p->trial.vd);
 }
 
-ORC_API long orc_pop_get_htrial(const orc_pop *p, int *vd, double *data) {
+static void build_htrial(orc_pop *p);
+
+ORC_API long orc_pop_get_htrial(orc_pop *p, int *vd, double *data) {
+    build_htrial(p);
     memcpy(vd, p->htrial.vd, sizeof(int) * (p->L + 1));
     if (data) mps_to_flat(&p->htrial, data);
     return flat_size(p->L, p->d, p->htrial.vd);
@@ -1201,8 +1204,8 @@ static void propagate_walker(orc_pop *p, int i, const double *u /* nC uniforms *
     apply_one_site_everywhere(p, m);
     p->ov[i] = set_overlap(m, &p->trial);
     for (int c = 0; c < p->nC; c++) {
-        for (int a = 0; a < n2; a++)
-            for (int b = 0; b < n2; b++) combos[a + n2 * b] = inner(&p->terms[(long)c * n2 * n2 + a + n2 * b], m);
+#pragma omp parallel for schedule(dynamic) num_threads(p->inner_threads) if (p->inner_threads > 1)
+        for (int q = 0; q < n2 * n2; q++) combos[q] = inner(&p->terms[(long)c * n2 * n2 + q], m);
         double norm = 0.0;
         for (int g = 0; g < N; g++) {
             double v = 0.0;
@@ -1268,6 +1271,19 @@ static void propagate_walker(orc_pop *p, int i, const double *u /* nC uniforms *
 ORC_API double orc_pop_propagate(orc_pop *p, const double *u, int *idx) {
     int nthr = p->nthreads;
 #ifdef _OPENMP
+    if (p->nw < nthr) {  /* few, large walkers: the threads share each walker's 16 overlaps */
+        double *combos = (double *)malloc(sizeof(double) * p->n2 * p->n2);
+        double *pdf = (double *)malloc(sizeof(double) * p->Ng);
+        double s = 0.0;
+        p->inner_threads = nthr;
+        for (int i = 0; i < p->nw; i++) {
+            propagate_walker(p, i, u + (long)i * p->nC, idx ? idx + (long)i * p->nC : NULL, combos, pdf);
+            s += p->wt[i];
+        }
+        p->inner_threads = 1;
+        free(combos); free(pdf);
+        return s;
+    }
     double *sums = (double *)calloc(nthr, sizeof(double));
 #pragma omp parallel num_threads(nthr)
     {
@@ -1302,22 +1318,24 @@ ORC_API double orc_pop_propagate(orc_pop *p, const double *u, int *idx) {
 /* <trial| H |phi> as sum over couplings of J * sum_a sgn_a <trial|S^a_i S^a_j|phi>
    minus h sum_k <trial|S^z_k|phi>: the same number as <H trial|phi>
    (HeisenbergMPO.h:98-103 convention; iSy enters with a minus sign). */
-static double h_insert(const orc_pop *p, const omps *phi) {
+static double h_insert(const orc_pop *p, const omps *phi, int nthr) {
     int d = p->d;
-    double *O = (double *)malloc(sizeof(double) * d * d);
     const int ops[3] = {OP_SX, OP_ISY, OP_SZ};
     const double sg[3] = {1.0, -1.0, 1.0};
     double tot = 0.0;
-    for (int c = 0; c < p->nC; c++)
-        for (int a = 0; a < 3; a++) {
-            omps t; mps_copy(&t, phi);
-            orc_operator(ops[a], d, O);
-            apply_local(&t, p->first[c], O);
-            apply_local(&t, p->second[c], O);
-            tot += sg[a] * p->Jc[c] * inner(&p->trial, &t);
-            mps_free(&t);
-        }
+#pragma omp parallel for schedule(dynamic) reduction(+ : tot) num_threads(nthr) if (nthr > 1)
+    for (int q = 0; q < 3 * p->nC; q++) {
+        const int c = q / 3, a = q % 3;
+        double O[16];
+        omps t; mps_copy(&t, phi);
+        orc_operator(ops[a], d, O);
+        apply_local(&t, p->first[c], O);
+        apply_local(&t, p->second[c], O);
+        tot += sg[a] * p->Jc[c] * inner(&p->trial, &t);
+        mps_free(&t);
+    }
     if (p->h != 0.0) {
+        double O[16];
         orc_operator(OP_SZ, d, O);
         for (int k = 0; k < p->L; k++) {
             omps t; mps_copy(&t, phi);
@@ -1326,23 +1344,41 @@ static double h_insert(const orc_pop *p, const omps *phi) {
             mps_free(&t);
         }
     }
-    free(O);
     return tot;
 }
 
 ORC_API void orc_pop_set_energy_mode(orc_pop *p, int mode) { p->emode = mode; }
 
+/* H|trial> with the reference's MPO (the lossless compression drops only
+   Schmidt values < 1e-13 and is skipped here), and with the minimal MPO. */
+static void build_htrial(orc_pop *p) {
+    if (p->htrial.t) return;
+    ompo M; M.L = p->L; M.d = p->d; M.J = p->Jfull; M.h = p->h;
+    M.dims = (int *)malloc(sizeof(int) * (p->L + 1));
+    mpo_dims(p->L, M.dims);
+    apply_mpo(&M, &p->trial, &p->htrial);
+    free(M.dims);
+}
+
+static void build_hmin(orc_pop *p) {
+    if (!p->hmin.t) apply_mpo_min(p->L, p->d, p->Jfull, p->h, &p->trial, &p->hmin);
+}
+
 /* MPSQMC2::EnergyFunctionAndHistory (MPSQMC2.cpp:688-747), non-MPI branch.
    Returns Omega; *EP the projected energy; EL (optional) per-walker energies. */
 ORC_API double orc_pop_energy(orc_pop *p, int step, int dofluct, double *EP, double *EL) {
     double s2 = 0.0, s1 = 0.0, num = 0.0, den = 0.0;
+    if (p->emode == 0) build_htrial(p);
+    if (p->emode == 2) build_hmin(p);
     double *e = p->EL;
+    /* few walkers: operator insertion shares each walker's terms among the threads */
+    const int inner_thr = (p->emode == 1 && p->nw < p->nthreads) ? p->nthreads : 1;
 #ifdef _OPENMP
-#pragma omp parallel for schedule(static) num_threads(p->nthreads)
+#pragma omp parallel for schedule(static) num_threads(p->nthreads) if (inner_thr == 1)
 #endif
     for (int i = 0; i < p->nw; i++)
         e[i] = (p->emode == 0 ? inner(&p->htrial, &p->w[i])
-                : (p->emode == 2 ? inner(&p->hmin, &p->w[i]) : h_insert(p, &p->w[i]))) / p->ov[i];
+                : (p->emode == 2 ? inner(&p->hmin, &p->w[i]) : h_insert(p, &p->w[i], inner_thr))) / p->ov[i];
     for (int i = 0; i < p->nw; i++) {
         num += p->wt[i] * e[i];
         den += p->wt[i];

@@ -25,8 +25,9 @@ def _engine(precision, L=12, D=8, n=32, seed=11, n_load=None, canon=3, dtau=0.01
     vd, data = random_trial(L, D, 5)
     e = Engine(s, vd, data, n_desired=n, capacity=4 * n, max_bond=D, precision=precision, seed=seed)
     e.set_canonical_interval(canon)
-    e.load_copies(vd, data, n if n_load is None else n_load)
-    e.prepare()
+    if n_load != 0:
+        e.load_copies(vd, data, n if n_load is None else n_load)
+        e.prepare()
     return e
 
 
@@ -108,25 +109,24 @@ def test_graph_replay_rejects_the_learned_proposal():
 
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
 def test_resume_with_population_growth(tmp_path, precision):
-    # 8 walkers loaded for 32 desired: population control splits every walker
-    # after the resume, so copies land in slots the snapshot never held
+    # 8 walkers loaded for 32 desired, snapshot before the first step: population
+    # control splits every walker after the resume, so copies land in slots the
+    # snapshot never held (their boundary environments must be set)
     a = _engine(precision, n_load=8)
-    a.run(1, 1)
     path = str(tmp_path / "w.state")
-    a.save_state(path, step=1)
-    n_saved = a.num_walkers()
-    a.run(2, 4)
-    ta = a.read_trace(2, 4)
+    a.save_state(path, step=0)
+    a.run(1, 4)
+    ta = a.read_trace(1, 4)
     sa = a.scalars()
     a.close()
-    b = _engine(precision, n_load=1)
-    assert b.load_state(path) == 1
-    assert b.num_walkers() == n_saved
-    b.run(2, 4)
-    tb = b.read_trace(2, 4)
+    b = _engine(precision, n_load=0)  # a fresh engine: no walkers loaded before the snapshot
+    assert b.load_state(path) == 0
+    assert b.num_walkers() == 8
+    b.run(1, 4)
+    tb = b.read_trace(1, 4)
     sb = b.scalars()
     b.close()
-    assert tb[-1, 1] > n_saved, "no walker copies after the resume"
+    assert tb[0, 1] == 8 and tb[1, 1] > 8, "no walker copies after the resume"
     np.testing.assert_array_equal(ta, tb)
     for k in ("w", "O", "eh", "EL"):
         np.testing.assert_array_equal(sa[k], sb[k])

@@ -261,31 +261,6 @@ def test_tile32_parity_chain16_fp32(canon):
     _run_parity(systems.chain(16), 16, 8, 17, steps=5, precision="fp32", tol=2e-4, check_states=False, canon=canon)
 
 
-def test_tile32_parity_chain32_d32_fp32():
-    # the benchmark shape (L = 32, D = 32) on a few walkers, one step
-    import os
-    s = systems.chain(32)
-    tvd, tdata = random_trial(32, 32, 19)
-    orc = oracle_population(s, tvd, tdata, threads=min(16, os.cpu_count() or 1))
-    orc.init_walkers(4, 32)
-    eng = Engine(s, tvd, tdata, n_desired=4, capacity=16, max_bond=32, precision="fp32")
-    assert eng.path == "tile32"
-    eng.load_copies(tvd, tdata, 4)
-    E_e = eng.prepare()
-    E_o = orc.energy(1, False)[0]
-    assert abs(E_e - E_o) < 1e-4 * abs(E_o)
-    u = Random(19).uniforms(4 * eng.nC)
-    sw_o, idx_o = orc.propagate(u)
-    sw_e = eng.propagate(u)
-    assert np.mean(idx_o == eng.indices(4)) > 0.95
-    wo, Oo, _ = orc.scalars()
-    se = eng.scalars()
-    assert rel(se["w"], wo) < 1e-3 and rel(se["O"], Oo) < 1e-3
-    EP_o, _, EL_o = orc.energy(1, True)
-    assert rel(eng.scalars()["EL"], EL_o) < 1e-3
-    eng.close()
-
-
 def test_parity_large_bond_fp64():
     # bond 128 in the middle of an L = 14 chain: the 256 x 128 FP64 matrix of the LQ
     # exceeds the LDS workspace, so its leading columns stay in global scratch

@@ -249,6 +249,23 @@ def test_debug_mps_fixture():
     assert E > -8.682473334398976  # variational: above the L = 20 ground state
 
 
+@pytest.mark.parametrize("name,E_var", [("L50D4.mps", -21.863425804367605), ("L50D8.mps", -21.966322890532627)])
+def test_l50_trial_fixtures(name, E_var):
+    # the reference's L = 50 trials (input/Heisenberg1D/L50D{4,8}.mps): stored
+    # normalised; ApplyMPO + InnerProduct give their variational energies with
+    # the reference's MPO and with the minimal one
+    L, Dt, d, vd, data = M.read_mps(os.path.join(GOLDEN, name))
+    assert L == 50 and d == 2
+    r, nd = oracle.left_normalize(L, 2, vd, data)
+    assert abs(abs(r) - 1.0) < 1e-10
+    sysm = systems.chain(L)
+    vo, H = oracle.apply_mpo(L, 2, sysm.J, 0.0, vd, data)
+    E = oracle.inner(L, 2, vd, data, vo, H)
+    assert abs(E - E_var) < 1e-10
+    v1, h1 = oracle.apply_mpo_min(L, 2, sysm.J, 0.0, vd, data)
+    assert abs(oracle.inner(L, 2, vd, data, v1, h1) - E_var) < 1e-10
+
+
 # --- population control ---------------------------------------------------------
 def test_population_control_counts():
     sysm = systems.chain(6)

@@ -1,6 +1,7 @@
 """Statistical agreement with the reference's own shipped runs
-(output/Heisenberg1D/ener_L50DT4DW{2,4}.txt: L = 50 open chain, trial
-input/Heisenberg1D/L50D4.mps, 1000 walkers, dtau = 0.01).  The reference runs
+(output/Heisenberg1D/ener_L50DT4DW{2,4,8}.txt and ener_L50DT8DW2.txt: L = 50
+open chain, trials input/Heisenberg1D/L50D4.mps and L50D8.mps, 1000 walkers,
+dtau = 0.01).  The reference runs
 are time-seeded, so agreement is within the statistical error of E_P,
 sqrt(Omega / N) per step (Omega from the same file)."""
 import os
@@ -27,9 +28,18 @@ def _reference_trace(name):
     return np.array(rows)  # step, N, E_P, E_T, Omega
 
 
-@pytest.mark.parametrize("DW,fname,precision", [(4, "ener_L50DT4DW4.txt", "fp64"), (2, "ener_L50DT4DW2.txt", "fp32")])
-def test_l50_reference_runs(DW, fname, precision):
-    L, Dt, d, vd, data = M.read_mps(os.path.join(GOLDEN, "L50D4.mps"))
+# E_var of the shipped trials (dense-free MPS energies, checked by the oracle tests)
+E_VAR = {"L50D4.mps": -21.863425804367605, "L50D8.mps": -21.966322890532627}
+
+
+@pytest.mark.parametrize("trial,DW,fname,precision", [
+    ("L50D4.mps", 4, "ener_L50DT4DW4.txt", "fp64"),
+    ("L50D4.mps", 2, "ener_L50DT4DW2.txt", "fp32"),
+    ("L50D4.mps", 8, "ener_L50DT4DW8.txt", "fp64"),   # DW > DT: walkers are trial copies
+    ("L50D8.mps", 2, "ener_L50DT8DW2.txt", "fp32"),   # trial D = 8 compressed to 2
+])
+def test_l50_reference_runs(trial, DW, fname, precision):
+    L, Dt, d, vd, data = M.read_mps(os.path.join(GOLDEN, trial))
     data, _ = M.left_normalize(vd, data)
     s = systems.chain(L)
     ref = _reference_trace(fname)
@@ -47,11 +57,11 @@ def test_l50_reference_runs(DW, fname, precision):
             O = -O
     else:
         wv, wd, O = vd, data, 1.0
-    eng = Engine(s, vd, data, n_desired=N, capacity=3 * N, max_bond=DW, precision=precision, seed=2013)
+    eng = Engine(s, vd, data, n_desired=N, capacity=3 * N, max_bond=min(DW, Dt), precision=precision, seed=2013)
     eng.load_copies(wv, wd, N, O=O)
     E0 = eng.prepare()
-    if DW == Dt:
-        assert abs(E0 - (-21.863425804367605)) < 1e-8   # E_var of L50D4.mps
+    if DW >= Dt:
+        assert abs(E0 - E_VAR[trial]) < (1e-8 if precision == "fp64" else 1e-5)
     eng.run(1, nsteps)
     tr = eng.read_trace(1, nsteps)
     eng.close()
