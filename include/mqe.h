@@ -115,6 +115,9 @@ int mqe_step(mqe_engine *e, int step);
  * otherwise) and is not timed by mqe_set_timing. */
 int mqe_run(mqe_engine *e, int first_step, int n_steps, int use_graph);
 
+/* mqe_run(e, first_step, n_steps, 1): the steps replayed from a captured HIP graph. */
+int mqe_step_graph(mqe_engine *e, int first_step, int n_steps);
+
 /* Copy the trace rows of steps first_step .. first_step+n_steps-1 (6 doubles each). */
 int mqe_read_trace(mqe_engine *e, int first_step, int n_steps, double *out);
 

@@ -61,6 +61,7 @@ SIGNATURES = {
     "mqe_population_control": (c_int, [ctypes.c_void_p, c_double, P(c_double), c_int, P(c_int), P(c_int)]),
     "mqe_step": (c_int, [ctypes.c_void_p, c_int]),
     "mqe_run": (c_int, [ctypes.c_void_p, c_int, c_int, c_int]),
+    "mqe_step_graph": (c_int, [ctypes.c_void_p, c_int, c_int]),
     "mqe_read_trace": (c_int, [ctypes.c_void_p, c_int, c_int, P(c_double)]),
     "mqe_sync": (c_int, [ctypes.c_void_p]),
     "mqe_kernel_path": (c_int, [ctypes.c_void_p]),

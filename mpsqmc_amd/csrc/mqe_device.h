@@ -153,16 +153,21 @@ __device__ __forceinline__ void block_inclusive_scan(int n, const double *x, dou
 // element (8(q/4) + 4(l/32) + q%4, l % 32) of the tile.
 // ---------------------------------------------------------------------------
 template <bool TA, bool TB>
-__device__ __forceinline__ void gemm_mfma(int m, int n, int k, const float *A, int lda, const float *B, int ldb,
-                                          float *C, int ldc, bool acc) {
+__device__ __forceinline__ void gemm_mfma(int m, int n, int k, const float *A0, int lda, const float *B0, int ldb,
+                                          float *C0, int ldc, bool acc, int nb = 1, long sA = 0, long sB = 0,
+                                          long sC = 0) {
     typedef float v16 __attribute__((ext_vector_type(16)));
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int r = lane & 31, h = lane >> 5;
     const int tm = (m + 31) >> 5, tn = (n + 31) >> 5;
-    // 16-byte loads along k need 16-byte aligned bases and leading dimensions
-    const bool avec = TA && ((((uintptr_t)A) | ((uintptr_t)lda * 4u)) & 15u) == 0;
-    const bool bvec = !TB && ((((uintptr_t)B) | ((uintptr_t)ldb * 4u)) & 15u) == 0;
-    for (int t = w; t < tm * tn; t += MQE_WAVES) {
+    // 16-byte loads along k need 16-byte aligned bases, leading dimensions and batch strides
+    const bool avec = TA && ((((uintptr_t)A0) | ((uintptr_t)lda * 4u) | ((uintptr_t)sA * 4u)) & 15u) == 0;
+    const bool bvec = !TB && ((((uintptr_t)B0) | ((uintptr_t)ldb * 4u) | ((uintptr_t)sB * 4u)) & 15u) == 0;
+    for (int tb = w; tb < nb * tm * tn; tb += MQE_WAVES) {
+        // tiles of all nb products of the batch are shared among the waves
+        const int bt = tb / (tm * tn), t = tb - bt * (tm * tn);
+        const float *A = A0 + (size_t)bt * sA, *B = B0 + (size_t)bt * sB;
+        float *C = C0 + (size_t)bt * sC;
         const int r0 = (t % tm) * 32, c0 = (t / tm) * 32;
         const int ar = r0 + r, bc = c0 + r;
         const bool aok = ar < m, bok = bc < n;
@@ -221,15 +226,19 @@ __device__ __forceinline__ void gemm_mfma(int m, int n, int k, const float *A, i
 // and accumulator slot q holds element (4(l/16) + q, l % 16) of the tile.
 // ---------------------------------------------------------------------------
 template <bool TA, bool TB>
-__device__ __forceinline__ void gemm_mfma16(int m, int n, int k, const float *A, int lda, const float *B, int ldb,
-                                            float *C, int ldc, bool acc) {
+__device__ __forceinline__ void gemm_mfma16(int m, int n, int k, const float *A0, int lda, const float *B0, int ldb,
+                                            float *C0, int ldc, bool acc, int nb = 1, long sA = 0, long sB = 0,
+                                            long sC = 0) {
     typedef float v4 __attribute__((ext_vector_type(4)));
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int r = lane & 15, g = lane >> 4;
     const int tm = (m + 15) >> 4, tn = (n + 15) >> 4;
-    const bool avec = TA && ((((uintptr_t)A) | ((uintptr_t)lda * 4u)) & 15u) == 0;
-    const bool bvec = !TB && ((((uintptr_t)B) | ((uintptr_t)ldb * 4u)) & 15u) == 0;
-    for (int t = w; t < tm * tn; t += MQE_WAVES) {
+    const bool avec = TA && ((((uintptr_t)A0) | ((uintptr_t)lda * 4u) | ((uintptr_t)sA * 4u)) & 15u) == 0;
+    const bool bvec = !TB && ((((uintptr_t)B0) | ((uintptr_t)ldb * 4u) | ((uintptr_t)sB * 4u)) & 15u) == 0;
+    for (int tb = w; tb < nb * tm * tn; tb += MQE_WAVES) {
+        const int bt = tb / (tm * tn), t = tb - bt * (tm * tn);
+        const float *A = A0 + (size_t)bt * sA, *B = B0 + (size_t)bt * sB;
+        float *C = C0 + (size_t)bt * sC;
         const int r0 = (t % tm) * 16, c0 = (t / tm) * 16;
         const int ar = r0 + r, bc = c0 + r;
         const bool aok = ar < m, bok = bc < n;
@@ -351,6 +360,44 @@ __device__ __forceinline__ void gemm_mfma_wide(int m, int n, int k, const float 
 // Output tile shapes of the FP32 matrix-core products (chosen per walker bond
 // bucket by the engine's tile table).
 enum : int { TILE_16 = 16, TILE_32 = 32, TILE_WIDE = 64 };
+
+// ---------------------------------------------------------------------------
+// nb products of one shape, C_b (+)= op(A_b) op(B_b) with A_b = A + b sA (and so
+// on), as one workgroup call: the output tiles of all of them are shared among
+// the four waves (small products keep every wave busy) and one barrier after
+// the call covers the batch.
+// ---------------------------------------------------------------------------
+template <bool TA, bool TB, typename T>
+__device__ __forceinline__ void gemm_batch(int nb, int m, int n, int k, const T *A, int lda, long sA, const T *B,
+                                           int ldb, long sB, T *C, int ldc, long sC, bool acc, int tile = TILE_32) {
+    if constexpr (sizeof(T) == sizeof(float)) {
+        if (m >= 8 && n >= 8 && k >= 4) {
+            if (tile == TILE_16)
+                gemm_mfma16<TA, TB>(m, n, k, (const float *)A, lda, (const float *)B, ldb, (float *)C, ldc, acc, nb,
+                                    sA, sB, sC);
+            else
+                gemm_mfma<TA, TB>(m, n, k, (const float *)A, lda, (const float *)B, ldb, (float *)C, ldc, acc, nb, sA,
+                                  sB, sC);
+            return;
+        }
+    }
+    const int mn = m * n;
+    for (int e = threadIdx.x; e < nb * mn; e += MQE_NT) {
+        const int bt = e / mn, x = e - bt * mn;
+        const int r = x % m, c = x / m;
+        const T *Ab = A + (size_t)bt * sA, *Bb = B + (size_t)bt * sB;
+        T *Cb = C + (size_t)bt * sC;
+        T s = acc ? Cb[r + ldc * c] : T(0);
+        const T *ap = TA ? (Ab + (size_t)lda * r) : (Ab + r);
+        const T *bp = TB ? (Bb + c) : (Bb + (size_t)ldb * c);
+        for (int q = 0; q < k; ++q) {
+            const T a = TA ? ap[q] : ap[(size_t)lda * q];
+            const T b = TB ? bp[(size_t)ldb * q] : bp[q];
+            s += a * b;
+        }
+        Cb[r + ldc * c] = s;
+    }
+}
 
 // ---------------------------------------------------------------------------
 // GEMM: C(m x n) (+)= op(A) op(B).  op(A) is m x k: A(r,q) = TA ? A[q + lda*r] : A[r + lda*q].

@@ -65,10 +65,14 @@ struct Ctl {
 
 // Re-orthonormalisation is staggered over the walker slots: each propagation
 // re-orthonormalises the slots of one phase, so every step costs the same and a
-// walker that keeps its slot does it every canon_every steps (a copy moved to a
-// new slot waits at most 2 canon_every - 2 steps).
-__host__ __device__ inline bool canon_slot(const Ctl &c, int slot) {
-    return c.canon_every <= 1 || (int)((slot + c.pc) % c.canon_every) == c.canon_every - 1;
+// walker that keeps its slot does it every canon_every steps.  A walker also
+// carries the number of propagations since its last re-orthonormalisation
+// (`since`, copied with it), and one that reaches canon_every - 1 does it
+// regardless of its slot's phase: copies moved to new slots, and copies of
+// copies, never go longer without it than a walker that stays put.
+__host__ __device__ inline bool canon_walker(const Ctl &c, int slot, int since) {
+    return c.canon_every <= 1 || since >= c.canon_every - 1 ||
+           (int)((slot + c.pc) % c.canon_every) == c.canon_every - 1;
 }
 
 // Read-only description of the system (device pointers).
@@ -106,6 +110,7 @@ struct Pool {
     unsigned long long seed;
     const Ctl *ctl;        // step control block
     const int *list;       // population positions in launch order (nullptr: block b = position b)
+    int *since;            // per slot: propagations since the last re-orthonormalisation
     int *status;
     char *scratch;
     long scratch_bytes;
@@ -125,7 +130,7 @@ struct Pool {
 
 // Scratch layout of one workgroup (sizes in elements).
 struct ScratchLayout {
-    long Y, G, GcA, GcB, W, V, N0, N1, TMP, LM, t_total;  // T region
+    long Y, Y4, G, GcA, GcB, W, V, N0, N1, TMP, LM, t_total;  // T region
     long AQ, RQ, TAU, PARTS, DOTS, PDF, CUM, d_total;     // double region
 };
 
@@ -134,6 +139,7 @@ __host__ __device__ inline ScratchLayout scratch_layout(int Dt, int Dw, int Ng) 
     const long Dm = Dt > Dw ? Dt : Dw;
     long o = 0;
     s.Y = o; o += Dm * 2 * Dm;
+    s.Y4 = o; o += 8L * Dt * Dw;  // the four carried channels times site k
     s.G = o; o += 4L * Dt * Dw;
     s.GcA = o; o += 4L * Dt * Dw;
     s.GcB = o; o += 4L * Dt * Dw;
@@ -208,13 +214,11 @@ __device__ void compute_V(const Ctx<T> &C, int j) {
     const int tj = C.tv[j], tn = C.tv[j + 1], wj = C.wv[j], wn = C.wv[j + 1];
     T *W = C.ts + C.sl.W, *V = C.ts + C.sl.V;
     const T *R = C.Rk(j + 1), *ph = C.site(j);
-    for (int t = 0; t < 2; ++t)
-        gemm<false, true, T>(tn, wj, wn, R, tn, ph + (size_t)t * wj * wn, wj, W + (size_t)t * tn * wj, tn, false, C.tile);
+    gemm_batch<false, true, T>(2, tn, wj, wn, R, tn, 0, ph, wj, (long)wj * wn, W, tn, (long)tn * wj, false, C.tile);
     __syncthreads();
     const T *Tj = C.tsite(j);
-    for (int tp = 0; tp < 2; ++tp)
-        gemm<false, false, T>(tj, 2 * wj, tn, Tj + (size_t)tp * tj * tn, tj, W, tn, V + (size_t)tp * tj * 2 * wj, tj,
-                              false, C.tile);
+    gemm_batch<false, false, T>(2, tj, 2 * wj, tn, Tj, tj, (long)tj * tn, W, tn, 0, V, tj, (long)tj * 2 * wj, false,
+                                C.tile);
     __syncthreads();
 }
 
@@ -223,18 +227,18 @@ __device__ void compute_V(const Ctx<T> &C, int j) {
 template <typename T>
 __device__ void carry_channels(const Ctx<T> &C, int k, const T *src, T *dst) {
     const int tk = C.tv[k], tn = C.tv[k + 1], wk = C.wv[k], wn = C.wv[k + 1];
-    T *Y = C.ts + C.sl.Y;
+    T *Y = C.ts + C.sl.Y4;  // Y_ch = src_ch phi_k (tk x 2 wn), the four channels one after the other
     const T *Tk = C.tsite(k);
-    for (int ch = 0; ch < 4; ++ch) {
-        gemm<false, false, T>(tk, 2 * wn, wk, src + (size_t)ch * tk * wk, tk, C.site(k), wk, Y, tk, false, C.tile);
-        __syncthreads();
-        T *d = dst + (size_t)ch * tn * wn;
-        gemm<true, false, T>(tn, wn, tk, Tk, tk, Y, tk, d, tn, false, C.tile);
-        // no barrier: the accumulating product has the same shape, so every
-        // thread re-reads only the output elements it wrote itself
-        gemm<true, false, T>(tn, wn, tk, Tk + (size_t)tk * tn, tk, Y + (size_t)tk * wn, tk, d, tn, true, C.tile);
-        __syncthreads();
-    }
+    const long sy = (long)tk * 2 * wn;
+    gemm_batch<false, false, T>(4, tk, 2 * wn, wk, src, tk, (long)tk * wk, C.site(k), wk, 0, Y, tk, sy, false, C.tile);
+    __syncthreads();
+    // dst_ch = T_k[0]^T Y_ch[0] + T_k[1]^T Y_ch[1]; no barrier between the two:
+    // the accumulating product has the same shape, so every thread re-reads
+    // only the output elements it wrote itself
+    gemm_batch<true, false, T>(4, tn, wn, tk, Tk, tk, 0, Y, tk, sy, dst, tn, (long)tn * wn, false, C.tile);
+    gemm_batch<true, false, T>(4, tn, wn, tk, Tk + (size_t)tk * tn, tk, 0, Y + (size_t)tk * wn, tk, sy, dst, tn,
+                               (long)tn * wn, true, C.tile);
+    __syncthreads();
 }
 
 // Two-site mixed tensor M[(s'+2t')*4 + (s+2t)] = <G(s',s), V(t',t)> at a bond
@@ -358,8 +362,7 @@ __device__ void advance_M(const Ctx<T> &C, int k) {
     T *X = C.ts + C.sl.Y;
     const T *ph = C.site(k), *Mn = C.Mk(k + 1);
     T *Mk = C.Mk(k);
-    for (int s = 0; s < 2; ++s)
-        gemm<false, false, T>(wk, wn, wn, ph + (size_t)s * wk * wn, wk, Mn, wn, X + (size_t)s * wk * wn, wk, false, C.tile);
+    gemm_batch<false, false, T>(2, wk, wn, wn, ph, wk, (long)wk * wn, Mn, wn, 0, X, wk, (long)wk * wn, false, C.tile);
     __syncthreads();
     gemm<false, true, T>(wk, wk, wn, X, wk, ph, wk, Mk, wk, false, C.tile);
     // same-shape accumulation: no barrier needed (each thread owns its outputs)
@@ -376,8 +379,7 @@ __device__ double norm2_at_M(const Ctx<T> &C, int k, const T *N) {
     const T *ph = C.site(k), *Mn = C.Mk(k + 1);
     gemm<false, false, T>(wk, 2 * wn, wk, N, wk, ph, wk, Z, wk, false, C.tile);
     __syncthreads();
-    for (int s = 0; s < 2; ++s)
-        gemm<false, false, T>(wk, wn, wn, Z + (size_t)s * wk * wn, wk, Mn, wn, Z2 + (size_t)s * wk * wn, wk, false, C.tile);
+    gemm_batch<false, false, T>(2, wk, wn, wn, Z, wk, (long)wk * wn, Mn, wn, 0, Z2, wk, (long)wk * wn, false, C.tile);
     __syncthreads();
     double acc = 0.0;
     for (int e = threadIdx.x; e < wk * 2 * wn; e += MQE_NT) acc += (double)ph[e] * (double)Z2[e];
@@ -440,8 +442,7 @@ __device__ void lq_site(const Ctx<T> &C, int k) {
     __syncthreads();
     // push Lmat into the left neighbour
     T *pl = C.site(k - 1), *tmp = C.ts + C.sl.TMP;
-    for (int s = 0; s < 2; ++s)
-        gemm<false, false, T>(wp, wk, wk, pl + (size_t)s * wp * wk, wp, LM, wk, tmp + (size_t)s * wp * wk, wp, false, C.tile);
+    gemm_batch<false, false, T>(2, wp, wk, wk, pl, wp, (long)wp * wk, LM, wk, 0, tmp, wp, (long)wp * wk, false, C.tile);
     __syncthreads();
     for (int e = threadIdx.x; e < 2 * wp * wk; e += MQE_NT) pl[e] = tmp[e];
     __syncthreads();
@@ -684,6 +685,7 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker_kernel(Sys S, Pool P, const 
         if (threadIdx.x == 0) {
             P.EL[slot] = e / O;  // stored overlap, MPSQMC2.cpp:447 / Walker.cpp:72-77
             P.O[slot] = On;
+            P.since[slot] = 0;
         }
         return;
     }
@@ -834,8 +836,10 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker_kernel(Sys S, Pool P, const 
     }
     // ---------------- local energy (Walker.cpp:72-77) ----------------
     double On, Ou;
-    const double e = energy_sweep(C, &On, &Ou, canon_slot(ctl, slot));
+    const bool canon = canon_walker(ctl, slot, P.since[slot]);
+    const double e = energy_sweep(C, &On, &Ou, canon);
     if (threadIdx.x == 0) {
+        P.since[slot] = canon ? 0 : P.since[slot] + 1;
         P.w[slot] = w;
         P.O[slot] = On;
         P.EL[slot] = e / Ou;
@@ -1083,7 +1087,8 @@ __global__ void advance_ctl_kernel(Ctl *ctl) {
 // population order.  Heavy walkers start in the first round of resident
 // workgroups instead of extending the last one.  key: scratch, one int per position.
 __global__ __launch_bounds__(RED_NT) void launch_list_kernel(const int *n_ptr, const int *order, const int *wvd,
-                                                             int L, const Ctl *ctl, int *key, int *list) {
+                                                             const int *since, int L, const Ctl *ctl, int *key,
+                                                             int *list) {
     __shared__ int shi[RED_NT / 64];
     __shared__ int cnt[16], base[16];
     __shared__ int carry;
@@ -1095,7 +1100,7 @@ __global__ __launch_bounds__(RED_NT) void launch_list_kernel(const int *n_ptr, c
         const int slot = order[b];
         int mb = 1;
         for (int k = 0; k <= L; ++k) mb = max(mb, wvd[(size_t)slot * (L + 1) + k]);
-        const int kk = (canon_slot(c, slot) ? 8 : 0) + bond_bucket(mb);
+        const int kk = (canon_walker(c, slot, since[slot]) ? 8 : 0) + bond_bucket(mb);
         key[b] = kk;
         atomicAdd(&cnt[kk], 1);
     }
@@ -1285,6 +1290,7 @@ __global__ __launch_bounds__(MQE_NT) void copy_walkers_kernel(Book B, Pool P, in
         P.O[dst] = P.O[src];
         P.eh[dst] = P.eh[src];
         P.EL[dst] = P.EL[src];
+        P.since[dst] = P.since[src];
         for (int k = 0; k <= L; ++k) P.wvd[(size_t)dst * (L + 1) + k] = P.wvd[(size_t)src * (L + 1) + k];
         if (P.bondE)
             for (int k = 0; k < P.nbe; ++k) P.bondE[(size_t)dst * P.nbe + k] = P.bondE[(size_t)src * P.nbe + k];
@@ -1431,6 +1437,7 @@ struct mqe_engine {
     Ctl *d_ctl = nullptr;                      // step control block (device)
     Ctl ctl_host{};                            // its last value set from the host
     int *d_list = nullptr, *d_key = nullptr;   // launch order of the walker kernel and its class keys
+    int *d_since = nullptr;                    // per slot: propagations since the last re-orthonormalisation
     long prop_draws = 0, pop_draws = 0;        // step words of mqe_propagate / mqe_population_control without host draws
     int n_host = 0;
     bool loaded = false, prepared = false;
@@ -1470,6 +1477,7 @@ static Pool make_pool(mqe_engine *e, const double *u) {
     P.seed = e->p.seed;
     P.ctl = e->d_ctl;
     P.list = nullptr;
+    P.since = e->d_since;
     P.status = e->d_status;
     P.scratch = e->d_scratch;
     P.scratch_bytes = e->scratch_bytes;
@@ -1540,8 +1548,8 @@ static int launch_walker_any(mqe_engine *e, const double *u, int flags) {
     const bool prop = (flags & F_PROP) != 0;
     if (prop) {
         ++e->prop_count;
-        hipLaunchKernelGGL(launch_list_kernel, dim3(1), dim3(RED_NT), 0, e->stream, e->d_n, e->d_order, e->d_wvd, e->L,
-                           e->d_ctl, e->d_key, e->d_list);
+        hipLaunchKernelGGL(launch_list_kernel, dim3(1), dim3(RED_NT), 0, e->stream, e->d_n, e->d_order, e->d_wvd,
+                           e->d_since, e->L, e->d_ctl, e->d_key, e->d_list);
         HIPCHK(hipGetLastError());
         P.list = e->d_list;
     }
@@ -1943,6 +1951,8 @@ int mqe_create(const mqe_params *p, const mqe_mps *trial, int n_couplings, const
     CK(hipMalloc((void **)&e->d_nw, sizeof(double) * cap));
     CK(hipMalloc((void **)&e->d_list, sizeof(int) * cap));
     CK(hipMalloc((void **)&e->d_key, sizeof(int) * cap));
+    CK(hipMalloc((void **)&e->d_since, sizeof(int) * cap));
+    CK(hipMemset(e->d_since, 0, sizeof(int) * cap));
     CK(hipMalloc((void **)&e->d_ctl, sizeof(Ctl)));
 
     Sys &S = e->sys;
@@ -2000,7 +2010,7 @@ void mqe_destroy(mqe_engine *e) {
                     e->d_n, e->d_idx, e->d_u, e->d_scratch, e->d_status, e->d_sums, e->d_trace, e->d_ncopy,
                     e->d_newpos, e->d_used, e->d_free, e->d_csrc, e->d_cdst, e->d_nextra, e->d_nnew, e->d_nused,
                     e->d_tmpout, e->d_bond, e->d_feat, e->d_pred, e->d_sX, e->d_sY, e->d_nsamp, e->d_plab,
-                    e->d_nw, e->d_list, e->d_key, e->d_ctl};
+                    e->d_nw, e->d_list, e->d_key, e->d_since, e->d_ctl};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     for (hipEvent_t ev : e->tev) (void)hipEventDestroy(ev);
@@ -2279,6 +2289,8 @@ int mqe_run(mqe_engine *e, int first_step, int n_steps, int use_graph) {
     return MQE_OK;
 }
 
+int mqe_step_graph(mqe_engine *e, int first_step, int n_steps) { return mqe_run(e, first_step, n_steps, 1); }
+
 int mqe_read_trace(mqe_engine *e, int first_step, int n_steps, double *out) {
     if (!e || !out || first_step < 1 || n_steps < 0 || n_steps > e->trace_cap) { set_error("mqe_read_trace: bad arguments"); return MQE_EINVAL; }
     HIPCHK(hipStreamSynchronize(e->stream));
@@ -2401,6 +2413,7 @@ int mqe_autotune(mqe_engine *e, int reps, int *tiles_out, double *ms_default, do
                                {e->d_w, sizeof(double) * cap, nullptr},     {e->d_O, sizeof(double) * cap, nullptr},
                                {e->d_eh, sizeof(double) * cap, nullptr},    {e->d_EL, sizeof(double) * cap, nullptr},
                                {e->d_order, sizeof(int) * cap, nullptr},    {e->d_n, sizeof(int), nullptr},
+                               {e->d_since, sizeof(int) * cap, nullptr},
                                {e->d_status, sizeof(int), nullptr}};
     int rc = MQE_OK;
     for (auto &pt : parts)
@@ -2811,7 +2824,7 @@ int mqe_save_state(mqe_engine *e, const char *path, long step) {
     if (!f) { set_error("mqe_save_state: cannot open the file"); return MQE_EINVAL; }
     StateHeader h;
     std::memset(&h, 0, sizeof(h));
-    std::memcpy(h.magic, "MQESTAT2", 8);
+    std::memcpy(h.magic, "MQESTAT3", 8);
     h.L = L;
     h.d = e->p.d;
     h.precision = e->p.precision;
@@ -2847,8 +2860,10 @@ int mqe_save_state(mqe_engine *e, const char *path, long step) {
             ok = ok && std::fwrite(buf.data(), tsz, pt.len, f) == (size_t)pt.len;
         }
         double sc[4];
+        int since = 0;
         const bool cp = hipMemcpy(vd.data(), e->d_wvd + sl * (L + 1), sizeof(int) * (L + 1), hipMemcpyDeviceToHost) ==
                             hipSuccess &&
+                        hipMemcpy(&since, e->d_since + sl, sizeof(int), hipMemcpyDeviceToHost) == hipSuccess &&
                         hipMemcpy(&sc[0], e->d_w + sl, sizeof(double), hipMemcpyDeviceToHost) == hipSuccess &&
                         hipMemcpy(&sc[1], e->d_O + sl, sizeof(double), hipMemcpyDeviceToHost) == hipSuccess &&
                         hipMemcpy(&sc[2], e->d_eh + sl, sizeof(double), hipMemcpyDeviceToHost) == hipSuccess &&
@@ -2856,6 +2871,7 @@ int mqe_save_state(mqe_engine *e, const char *path, long step) {
         if (!cp) return fail("mqe_save_state: device copy failed");
         ok = ok && std::fwrite(vd.data(), sizeof(int), L + 1, f) == (size_t)(L + 1);
         ok = ok && std::fwrite(sc, sizeof(double), 4, f) == 4;
+        ok = ok && std::fwrite(&since, sizeof(int), 1, f) == 1;
         if (e->tile32) {
             if (hipMemcpy(bond.data(), e->d_bond + sl * (nC + 1), sizeof(double) * (nC + 1), hipMemcpyDeviceToHost) !=
                     hipSuccess ||
@@ -2876,7 +2892,7 @@ int mqe_load_state(mqe_engine *e, const char *path, long *step) {
     FILE *f = std::fopen(path, "rb");
     if (!f) { set_error("mqe_load_state: cannot open the file"); return MQE_EINVAL; }
     StateHeader h;
-    if (std::fread(&h, sizeof(h), 1, f) != 1 || std::memcmp(h.magic, "MQESTAT2", 8) != 0) {
+    if (std::fread(&h, sizeof(h), 1, f) != 1 || std::memcmp(h.magic, "MQESTAT3", 8) != 0) {
         std::fclose(f);
         set_error("mqe_load_state: not a walker-state file");
         return MQE_EINVAL;
@@ -2921,9 +2937,12 @@ int mqe_load_state(mqe_engine *e, const char *path, long *step) {
                                    hipMemcpyHostToDevice) == hipSuccess;
         }
         double sc[4];
+        int since = 0;
         ok = ok && std::fread(vd.data(), sizeof(int), L + 1, f) == (size_t)(L + 1);
         ok = ok && std::fread(sc, sizeof(double), 4, f) == 4;
+        ok = ok && std::fread(&since, sizeof(int), 1, f) == 1 && since >= 0;
         if (!ok) break;
+        dev = dev && hipMemcpy(e->d_since + sl, &since, sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
         for (int k = 0; k <= L; ++k)
             if (vd[k] < 1 || vd[k] > std::max(e->wmax[k], 1)) ok = false;
         if (!ok) break;

@@ -683,6 +683,7 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
         if (threadIdx.x == 0) {
             P.EL[slot] = e / O;
             P.O[slot] = On;
+            P.since[slot] = 0;
         }
         return;
     }
@@ -1013,8 +1014,10 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
     __syncthreads();
     // ---------------- local energy (Walker.cpp:72-77) ----------------
     double On, Ou;
-    const double e = energy_sweep32(C, pc, &On, &Ou, canon_slot(ctl, slot));
+    const bool canon = canon_walker(ctl, slot, P.since[slot]);
+    const double e = energy_sweep32(C, pc, &On, &Ou, canon);
     if (threadIdx.x == 0) {
+        P.since[slot] = canon ? 0 : P.since[slot] + 1;
         P.w[slot] = wt;
         P.O[slot] = On;
         P.EL[slot] = e / Ou;

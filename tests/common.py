@@ -105,3 +105,21 @@ def dense_to_mps(psi, L, D, d=2):
     A = rest.reshape(dl, d, 1).transpose(1, 0, 2)
     tensors.append(A)
     return M.from_site_tensors(tensors)
+
+
+def random_gauge(vd, data, seed, eps=0.3):
+    """The same state in a random gauge: A_k <- A_k G_k, A_{k+1} <- G_k^-1 A_{k+1}
+    on every inner bond (G_k = 1 + eps * gaussian).  Canonical MPS have exactly
+    degenerate two-site singular values near the chain ends, where the
+    reference's CompressState truncation picks an SVD-implementation-dependent
+    subspace; in a generic gauge the truncation is well defined."""
+    rng = np.random.default_rng(seed)
+    L = len(vd) - 1
+    T = [M.site_tensor(vd, data, k) for k in range(L)]  # A[s, l, r]
+    for k in range(1, L):
+        D = int(vd[k])
+        G = np.eye(D) + eps * rng.normal(size=(D, D)) / np.sqrt(D)
+        Gi = np.linalg.inv(G)
+        T[k - 1] = np.einsum("slr,rq->slq", T[k - 1], G)
+        T[k] = np.einsum("ql,slr->sqr", Gi, T[k])
+    return M.from_site_tensors(T)

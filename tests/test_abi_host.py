@@ -94,6 +94,36 @@ def test_compress_matches_oracle():
     assert abs(ov / np.sqrt(n1 * n2) - 1.0) < 1e-10
 
 
+def test_compress_gauged_trial_matches_oracle():
+    # a left-normalised trial in a random gauge (the device recompression tests
+    # compress such states): the same state, generic two-site singular values,
+    # so the host restatement and the oracle keep the same states
+    from .common import random_trial, random_gauge
+    tvd, tdata = random_trial(10, 16, 3)
+    # canonical form: the first sweep's two-site matrices near the left end have
+    # equal singular values (sqrt(2) four times at the second bond), so a
+    # truncation to fewer states there is a convention of the SVD
+    A = [M.site_tensor(tvd, tdata, k) for k in range(3)]  # A[s, l, r]
+
+    def two_site_svd(a, b):
+        dl, dr = a.shape[1], b.shape[2]
+        th = np.einsum("slm,tmr->lsrt", a, b).reshape(dl * 2, dr * 2)
+        return np.linalg.svd(th, full_matrices=False)
+    U, S, VT = two_site_svd(A[0], A[1])  # bond 1, moving right: S VT into site 1
+    A1 = (S[:, None] * VT).reshape(len(S), A[1].shape[2], 2).transpose(2, 0, 1)
+    _, S2, _ = two_site_svd(A1, A[2])
+    assert np.allclose(S2, np.sqrt(2.0), atol=1e-12) and len(S2) == 4
+    gvd, gdata = random_gauge(tvd, tdata, 4)
+    assert np.allclose(M.to_dense(gvd, gdata), M.to_dense(tvd, tdata))
+    for D in (3, 4, 8):
+        v1, c1 = M.compress(gvd, gdata, D)
+        v2, c2, _ = oracle.compress(10, 2, gvd, gdata, D)
+        assert list(v1) == list(v2)
+        n1 = oracle.inner(10, 2, v1, c1, v1, c1)
+        n2 = oracle.inner(10, 2, v2, c2, v2, c2)
+        assert abs(oracle.inner(10, 2, v1, c1, v2, c2) / np.sqrt(n1 * n2) - 1.0) < 1e-10
+
+
 def test_host_rng_contract():
     r = Random(20131015)
     u = r.uniforms(1000)

@@ -15,7 +15,7 @@ from mpsqmc_amd import systems
 from mpsqmc_amd.engine import Engine
 from mpsqmc_amd.rng import Random
 
-from .common import random_trial, rel
+from .common import random_gauge, random_trial, rel
 
 pytestmark = [pytest.mark.gpu, pytest.mark.usefixtures("require_gpu")]
 
@@ -27,7 +27,8 @@ def _setup(precision, seed=5):
     tvd, tdata = random_trial(s.L, 64, seed)
     dws = [16, 32, 48, 64, 16, 32, 48, 64]
     e = Engine(s, tvd, tdata, n_desired=8, capacity=32, max_bond=64, precision=precision, seed=seed)
-    e.load_copies(tvd, tdata, 8)
+    gvd, gdata = random_gauge(tvd, tdata, seed + 1)  # generic gauge: a well-defined truncation
+    e.load_copies(gvd, gdata, 8)
     e.compress_walkers([D if D < 64 else 0 for D in dws])
     return e
 

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 from mpsqmc_amd import systems
-from mpsqmc_amd._lib import MqeError
+from mpsqmc_amd._lib import MqeError, check, lib
 from mpsqmc_amd.engine import Engine
 
 from .common import random_trial
@@ -86,7 +86,7 @@ def test_graph_replay_equals_eager(precision):
     b.sync()
     tb1 = b.read_trace(1, k)
     b.run(k + 1, 3)            # eager steps between two replays of the same graph
-    b.run(k + 4, k, use_graph=True)
+    check(lib().mqe_step_graph(b.h, k + 4, k))
     tb = b.read_trace(1, 2 * k + 3)
     sb = b.scalars()
     b.close()
@@ -149,3 +149,24 @@ def test_capacity_overflow_leaves_the_population():
     with pytest.raises(MqeError):
         e.propagate(None)
     e.close()
+
+
+def test_long_walk_every_walker_re_orthonormalises():
+    # the benchmark walk (L = 32, D = 32 DMRG trial, 2048 walkers) over 1920 steps:
+    # staggered by slot alone, a line of copies moved to new slots could go
+    # without re-orthonormalisation for hundreds of steps, and its FP32 gauge
+    # drifted until a PDF norm vanished near step 1800; the per-walker counter
+    # bounds every walker's interval to the default 64
+    from mpsqmc_amd import mps as M
+    s = systems.chain(32, dtau=0.01)
+    vd, data, _ = M.dmrg(s, 32, max_sweeps=6, tol=1e-8, seed=20131015)
+    e = Engine(s, vd, data, n_desired=2048, capacity=6144, max_bond=32, precision="fp32", seed=20131015)
+    e.load_copies(vd, data, 2048)
+    e.prepare()
+    for first in range(1, 1921, 320):
+        e.run(first, 320)
+        e.sync()  # raises on a zero PDF norm or a non-finite weight
+    tr = e.read_trace(1, 1920)
+    e.close()
+    assert np.all(np.isfinite(tr))
+    assert abs(tr[-200:, 2].mean() - (-13.99732)) < 1e-4
