@@ -184,6 +184,17 @@ int mqe_read_walker(mqe_engine *e, int i, int *vd, double *data, long *size);
  * batched over the population; other may have any bond dimensions. */
 int mqe_overlaps(mqe_engine *e, const mqe_mps *other, double *out);
 
+/* mqe_overlaps without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Its workspace (the
+ * other MPS and one transfer-matrix workspace per walker of a chunk) is a
+ * fixed 64 MiB device array per GPU, held until the process ends; walkers run
+ * in chunks of as many workspaces as fit.  MQE_ECAPACITY when the other MPS and
+ * one walker's workspace (4 Dmax^2 elements, Dmax the largest bond of the
+ * walkers and of `other`) exceed it.  Every call on one GPU uses the same
+ * array: calls that overlap on one GPU (two host threads, two engines) race. */
+int mqe_overlaps_noalloc(mqe_engine *e, const mqe_mps *other, double *out);
+
 /* Device time of the walker kernel of the last mqe_propagate in milliseconds
  * (HIP events on the engine's stream), and the population of the last
  * propagation (mqe_propagate or a step of mqe_run). */

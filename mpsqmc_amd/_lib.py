@@ -72,6 +72,7 @@ SIGNATURES = {
     "mqe_read_indices": (c_int, [ctypes.c_void_p, P(c_int)]),
     "mqe_read_walker": (c_int, [ctypes.c_void_p, c_int, P(c_int), P(c_double), P(c_long)]),
     "mqe_overlaps": (c_int, [ctypes.c_void_p, P(MqeMps), P(c_double)]),
+    "mqe_overlaps_noalloc": (c_int, [ctypes.c_void_p, P(MqeMps), P(c_double)]),
     "mqe_last_kernel_ms": (c_int, [ctypes.c_void_p, P(c_double), P(c_int)]),
     "mqe_set_timing": (c_int, [ctypes.c_void_p, c_int]),
     "mqe_kernel_time": (c_int, [ctypes.c_void_p, P(c_double), P(c_int)]),

@@ -12,7 +12,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmqe.so")
 
-SOURCES = ["mqe_engine.hip", "mqe_mlp.hip", "mqe_host.cpp", "mqe_dmrg.cpp", "mpsqmc.cpp"]
+SOURCES = ["mqe_engine.hip", "mqe_mlp.hip", "mqe_noalloc.hip", "mqe_host.cpp", "mqe_dmrg.cpp", "mpsqmc.cpp"]
 WALK = os.path.join(HERE, "mpsqmc_walk")
 TOOL = os.path.join(HERE, "mpsqmc_tool")
 
