@@ -141,214 +141,104 @@ __device__ __forceinline__ void block_inclusive_scan(int n, const double *x, dou
 }
 
 // ---------------------------------------------------------------------------
-// FP32 GEMM on the matrix cores: C(m x n) (+)= op(A) op(B), whole workgroup.
-// The output is cut into 32 x 32 tiles, one wave per tile at a time; each tile
-// is a chain of v_mfma_f32_32x32x2f32 over k with operands read straight from
-// (L2-resident) memory, zero outside the matrices.  Within a 16-deep slice of
-// k starting at k0, MFMA u of the chain takes kk = k0 + 8(l/32) + u in lane l:
-// lane l holds A(l % 32, kk) and B(kk, l % 32), so each lane's eight values of
-// an operand stored contiguously along k are one 32-byte run (two 16-byte
-// loads when aligned), and an operand stored contiguously along its other
-// index is read coalesced across the lanes.  Accumulator slot q of lane l is
-// element (8(q/4) + 4(l/32) + q%4, l % 32) of the tile.
+// FP32 GEMM on the matrix cores: C(m x n) = sum_{s < nseg} op(A_s) op(B_s)
+// (nseg = 1 or 2 terms of one shape and leading dimensions), for nb products
+// at once (A_s + b sA and so on), whole workgroup.  The output is cut into
+// tiles, one wave per tile at a time; the tiles of all nb products are shared
+// among the four waves.  Each tile is a chain of MFMAs over the k slices of
+// its terms.  A lane's operand values of a slice are one run along k, loaded
+// straight into the MFMA operand registers without predication: rows and
+// columns beyond the matrix are clamped to its last row / column (they only
+// feed output elements that are not stored), so the wait for a slice's data
+// sits right before its MFMAs, and the loads of the wave's next slice (of this
+// tile or the first of its next tile) are issued before the MFMAs of the
+// current one.  k not a multiple of the slice depth (16) takes a predicated
+// path with zeros outside the matrices.  acc: C += instead of C =.
 // ---------------------------------------------------------------------------
-template <bool TA, bool TB>
-__device__ __forceinline__ void gemm_mfma(int m, int n, int k, const float *A0, int lda, const float *B0, int ldb,
-                                          float *C0, int ldc, bool acc, int nb = 1, long sA = 0, long sB = 0,
-                                          long sC = 0) {
-    typedef float v16 __attribute__((ext_vector_type(16)));
-    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int r = lane & 31, h = lane >> 5;
-    const int tm = (m + 31) >> 5, tn = (n + 31) >> 5;
-    // 16-byte loads along k need 16-byte aligned bases, leading dimensions and batch strides
-    const bool avec = TA && ((((uintptr_t)A0) | ((uintptr_t)lda * 4u) | ((uintptr_t)sA * 4u)) & 15u) == 0;
-    const bool bvec = !TB && ((((uintptr_t)B0) | ((uintptr_t)ldb * 4u) | ((uintptr_t)sB * 4u)) & 15u) == 0;
-    for (int tb = w; tb < nb * tm * tn; tb += MQE_WAVES) {
-        // tiles of all nb products of the batch are shared among the waves
-        const int bt = tb / (tm * tn), t = tb - bt * (tm * tn);
-        const float *A = A0 + (size_t)bt * sA, *B = B0 + (size_t)bt * sB;
-        float *C = C0 + (size_t)bt * sC;
-        const int r0 = (t % tm) * 32, c0 = (t / tm) * 32;
-        const int ar = r0 + r, bc = c0 + r;
-        const bool aok = ar < m, bok = bc < n;
-        v16 c;
+
+// Run of NV operand values at k = kb..kb+NV-1 of one row of op(X), inside the
+// matrix: KC -- stored along k (p[kk]), else p[ld kk].  vec: KC with p + kb
+// 16-byte aligned.
+template <bool KC, int NV>
+__device__ __forceinline__ void load_run(const float *p, int ld, int kb, bool vec, float (&v)[NV]) {
+    if (KC && vec) {
+        const float4 *q = (const float4 *)(p + kb);
 #pragma unroll
-        for (int q = 0; q < 16; ++q) c[q] = 0.f;
-        for (int k0 = 0; k0 < k; k0 += 16) {
-            const int kb = k0 + 8 * h;
-            const bool full = k0 + 16 <= k;
-            float a[8], b[8];
-            if (avec && full) {
-                const float4 *p = (const float4 *)(A + (size_t)lda * (aok ? ar : 0) + kb);
-                const float4 x = p[0], y = p[1];
-                a[0] = aok ? x.x : 0.f; a[1] = aok ? x.y : 0.f; a[2] = aok ? x.z : 0.f; a[3] = aok ? x.w : 0.f;
-                a[4] = aok ? y.x : 0.f; a[5] = aok ? y.y : 0.f; a[6] = aok ? y.z : 0.f; a[7] = aok ? y.w : 0.f;
-            } else {
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const int kk = kb + u;
-                    a[u] = (aok && kk < k) ? (TA ? A[kk + (size_t)lda * ar] : A[ar + (size_t)lda * kk]) : 0.f;
-                }
-            }
-            if (bvec && full) {
-                const float4 *p = (const float4 *)(B + (size_t)ldb * (bok ? bc : 0) + kb);
-                const float4 x = p[0], y = p[1];
-                b[0] = bok ? x.x : 0.f; b[1] = bok ? x.y : 0.f; b[2] = bok ? x.z : 0.f; b[3] = bok ? x.w : 0.f;
-                b[4] = bok ? y.x : 0.f; b[5] = bok ? y.y : 0.f; b[6] = bok ? y.z : 0.f; b[7] = bok ? y.w : 0.f;
-            } else {
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const int kk = kb + u;
-                    b[u] = (bok && kk < k) ? (TB ? B[bc + (size_t)ldb * kk] : B[kk + (size_t)ldb * bc]) : 0.f;
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < 8; ++u) c = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u], b[u], c, 0, 0, 0);
+        for (int j = 0; j < NV / 4; ++j) {
+            const float4 x = q[j];
+            v[4 * j] = x.x;
+            v[4 * j + 1] = x.y;
+            v[4 * j + 2] = x.z;
+            v[4 * j + 3] = x.w;
         }
+    } else {
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int rr = r0 + 8 * (q >> 2) + 4 * h + (q & 3);
-            if (rr < m && bok) {
-                float v = c[q];
-                if (acc) v += C[rr + (size_t)ldc * bc];
-                C[rr + (size_t)ldc * bc] = v;
-            }
-        }
+        for (int u = 0; u < NV; ++u) v[u] = KC ? p[kb + u] : p[(size_t)ld * (kb + u)];
     }
 }
 
-// ---------------------------------------------------------------------------
-// The same product with 16 x 16 output tiles (v_mfma_f32_16x16x4f32): small
-// operands (bonds of 16 or 48) waste less of each tile and spread over all
-// four waves.  Within a 16-deep slice of k starting at k0, MFMA u takes
-// kk = k0 + 4(l/16) + u in lane l: lane l holds A(l % 16, kk) and B(kk, l % 16)
-// (four consecutive values of an operand stored along k: one 16-byte load),
-// and accumulator slot q holds element (4(l/16) + q, l % 16) of the tile.
-// ---------------------------------------------------------------------------
-template <bool TA, bool TB>
-__device__ __forceinline__ void gemm_mfma16(int m, int n, int k, const float *A0, int lda, const float *B0, int ldb,
-                                            float *C0, int ldc, bool acc, int nb = 1, long sA = 0, long sB = 0,
-                                            long sC = 0) {
-    typedef float v4 __attribute__((ext_vector_type(4)));
-    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int r = lane & 15, g = lane >> 4;
-    const int tm = (m + 15) >> 4, tn = (n + 15) >> 4;
-    const bool avec = TA && ((((uintptr_t)A0) | ((uintptr_t)lda * 4u) | ((uintptr_t)sA * 4u)) & 15u) == 0;
-    const bool bvec = !TB && ((((uintptr_t)B0) | ((uintptr_t)ldb * 4u) | ((uintptr_t)sB * 4u)) & 15u) == 0;
-    for (int tb = w; tb < nb * tm * tn; tb += MQE_WAVES) {
-        const int bt = tb / (tm * tn), t = tb - bt * (tm * tn);
-        const float *A = A0 + (size_t)bt * sA, *B = B0 + (size_t)bt * sB;
-        float *C = C0 + (size_t)bt * sC;
-        const int r0 = (t % tm) * 16, c0 = (t / tm) * 16;
-        const int ar = r0 + r, bc = c0 + r;
-        const bool aok = ar < m, bok = bc < n;
-        v4 c = {0.f, 0.f, 0.f, 0.f};
-        for (int k0 = 0; k0 < k; k0 += 16) {
-            const int kb = k0 + 4 * g;
-            const bool full = k0 + 16 <= k;
-            float a[4], b[4];
-            if (avec && full) {
-                const float4 x = *(const float4 *)(A + (size_t)lda * (aok ? ar : 0) + kb);
-                a[0] = aok ? x.x : 0.f; a[1] = aok ? x.y : 0.f; a[2] = aok ? x.z : 0.f; a[3] = aok ? x.w : 0.f;
-            } else {
+// The same run with zeros at k >= kmax.
+template <bool KC, int NV>
+__device__ __forceinline__ void load_run_pred(const float *p, int ld, int kb, int kmax, float (&v)[NV]) {
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int kk = kb + u;
-                    a[u] = (aok && kk < k) ? (TA ? A[kk + (size_t)lda * ar] : A[ar + (size_t)lda * kk]) : 0.f;
-                }
-            }
-            if (bvec && full) {
-                const float4 x = *(const float4 *)(B + (size_t)ldb * (bok ? bc : 0) + kb);
-                b[0] = bok ? x.x : 0.f; b[1] = bok ? x.y : 0.f; b[2] = bok ? x.z : 0.f; b[3] = bok ? x.w : 0.f;
-            } else {
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int kk = kb + u;
-                    b[u] = (bok && kk < k) ? (TB ? B[bc + (size_t)ldb * kk] : B[kk + (size_t)ldb * bc]) : 0.f;
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], b[u], c, 0, 0, 0);
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int rr = r0 + 4 * g + q;
-            if (rr < m && bok) {
-                float v = c[q];
-                if (acc) v += C[rr + (size_t)ldc * bc];
-                C[rr + (size_t)ldc * bc] = v;
-            }
-        }
+    for (int u = 0; u < NV; ++u) {
+        const int kk = kb + u;
+        v[u] = kk < kmax ? (KC ? p[kk] : p[(size_t)ld * kk]) : 0.f;
     }
 }
 
-// 32 x 64 output blocks: each wave computes two neighbouring 32 x 32 tiles that
-// share the A panel (half the A reads of gemm_mfma for wide products).
-template <bool TA, bool TB>
-__device__ __forceinline__ void gemm_mfma_wide(int m, int n, int k, const float *A, int lda, const float *B,
-                                               int ldb, float *C, int ldc, bool acc) {
-    typedef float v16 __attribute__((ext_vector_type(16)));
-    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int r = lane & 31, h = lane >> 5;
-    const int tm = (m + 31) >> 5, tn = (n + 63) >> 6;
-    const bool avec = TA && ((((uintptr_t)A) | ((uintptr_t)lda * 4u)) & 15u) == 0;
-    const bool bvec = !TB && ((((uintptr_t)B) | ((uintptr_t)ldb * 4u)) & 15u) == 0;
-    for (int t = w; t < tm * tn; t += MQE_WAVES) {
-        const int r0 = (t % tm) * 32, c0 = (t / tm) * 64;
-        const bool two = c0 + 32 < n;  // wave-uniform
-        const int ar = r0 + r;
-        const bool aok = ar < m;
-        v16 c[2];
+// 16-byte loads along k need 16-byte aligned bases, leading dimension and batch stride
+__device__ __forceinline__ bool aligned16(const float *X0, const float *X1, int ld, long s) {
+    return ((((uintptr_t)X0) | ((uintptr_t)X1) | ((uintptr_t)ld * 4u) | ((uintptr_t)s * 4u)) & 15u) == 0;
+}
+
+// TS x TS output tiles (TS = 32: v_mfma_f32_32x32x2f32, TS = 16:
+// v_mfma_f32_16x16x4f32), NW adjacent tiles along n per wave sharing the A
+// run (NW = 2 only with TS = 32).  16-deep slices; within the slice at k0,
+// MFMA u takes kk = k0 + KL (l / TS) + u in lane l (KL = 8 for TS = 32, 4 for
+// TS = 16), which holds A(l % TS, kk) and B(kk, l % TS).  Accumulator slot q
+// of lane l is element (8(q/4) + 4(l/32) + q%4, l % 32) of a 32 x 32 tile,
+// (4(l/16) + q, l % 16) of a 16 x 16 tile.
+template <int TS>
+struct TileGeom {
+    static constexpr int KL = TS == 32 ? 8 : 4;  // k values per lane and slice
+    static constexpr int NACC = TS == 32 ? 16 : 4;
+};
+
+// Lane operand pointers of tile tb (of all nb products) and term seg: the
+// lane's row of op(A) and its NW columns of op(B), clamped into the matrices.
+template <bool TA, bool TB, int TS, int NW>
+__device__ __forceinline__ void tile_ptrs(int tb, int seg, int m, int n, int tm, int tn, const float *A0,
+                                          const float *A1, int lda, const float *B0, const float *B1, int ldb, long sA,
+                                          long sB, int r, const float *&pa, const float *(&pb)[NW]) {
+    const int bt = tb / (tm * tn), t = tb - bt * (tm * tn);
+    const int ar = min((t % tm) * TS + r, m - 1);
+    const float *A = (seg ? A1 : A0) + (size_t)bt * sA, *B = (seg ? B1 : B0) + (size_t)bt * sB;
+    pa = TA ? A + (size_t)lda * ar : A + ar;
 #pragma unroll
-        for (int q = 0; q < 16; ++q) { c[0][q] = 0.f; c[1][q] = 0.f; }
-        for (int k0 = 0; k0 < k; k0 += 16) {
-            const int kb = k0 + 8 * h;
-            const bool full = k0 + 16 <= k;
-            float a[8];
-            if (avec && full) {
-                const float4 *p = (const float4 *)(A + (size_t)lda * (aok ? ar : 0) + kb);
-                const float4 x = p[0], y = p[1];
-                a[0] = aok ? x.x : 0.f; a[1] = aok ? x.y : 0.f; a[2] = aok ? x.z : 0.f; a[3] = aok ? x.w : 0.f;
-                a[4] = aok ? y.x : 0.f; a[5] = aok ? y.y : 0.f; a[6] = aok ? y.z : 0.f; a[7] = aok ? y.w : 0.f;
-            } else {
+    for (int q = 0; q < NW; ++q) {
+        const int bc = min((t / tm) * TS * NW + TS * q + r, n - 1);
+        pb[q] = TB ? B + bc : B + (size_t)ldb * bc;
+    }
+}
+
+template <int TS, int NW>
+__device__ __forceinline__ void tile_store(float *C0, int ldc, long sC, int tb, int m, int n, int tm, int tn, int r,
+                                           int h, bool acc,
+                                           const float (&c)[NW][TileGeom<TS>::NACC]) {
+    constexpr int NACC = TileGeom<TS>::NACC;
+    const int bt = tb / (tm * tn), t = tb - bt * (tm * tn);
+    const int r0 = (t % tm) * TS;
+    float *C = C0 + (size_t)bt * sC;
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const int kk = kb + u;
-                    a[u] = (aok && kk < k) ? (TA ? A[kk + (size_t)lda * ar] : A[ar + (size_t)lda * kk]) : 0.f;
-                }
-            }
+    for (int q = 0; q < NW; ++q) {
+        const int bc = (t / tm) * TS * NW + TS * q + r;
+        if (bc < n) {
 #pragma unroll
-            for (int half = 0; half < 2; ++half) {
-                if (half == 1 && !two) break;
-                const int bc = c0 + 32 * half + r;
-                const bool bok = bc < n;
-                float b[8];
-                if (bvec && full) {
-                    const float4 *p = (const float4 *)(B + (size_t)ldb * (bok ? bc : 0) + kb);
-                    const float4 x = p[0], y = p[1];
-                    b[0] = bok ? x.x : 0.f; b[1] = bok ? x.y : 0.f; b[2] = bok ? x.z : 0.f; b[3] = bok ? x.w : 0.f;
-                    b[4] = bok ? y.x : 0.f; b[5] = bok ? y.y : 0.f; b[6] = bok ? y.z : 0.f; b[7] = bok ? y.w : 0.f;
-                } else {
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) {
-                        const int kk = kb + u;
-                        b[u] = (bok && kk < k) ? (TB ? B[bc + (size_t)ldb * kk] : B[kk + (size_t)ldb * bc]) : 0.f;
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < 8; ++u) c[half] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u], b[u], c[half], 0, 0, 0);
-            }
-        }
-#pragma unroll
-        for (int half = 0; half < 2; ++half) {
-            if (half == 1 && !two) break;
-            const int bc = c0 + 32 * half + r;
-            if (bc >= n) continue;
-#pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                const int rr = r0 + 8 * (q >> 2) + 4 * h + (q & 3);
+            for (int e = 0; e < NACC; ++e) {
+                const int rr = TS == 32 ? r0 + 8 * (e >> 2) + 4 * h + (e & 3) : r0 + 4 * h + e;
                 if (rr < m) {
-                    float v = c[half][q];
+                    float v = c[q][e];
                     if (acc) v += C[rr + (size_t)ldc * bc];
                     C[rr + (size_t)ldc * bc] = v;
                 }
@@ -357,9 +247,185 @@ __device__ __forceinline__ void gemm_mfma_wide(int m, int n, int k, const float 
     }
 }
 
+template <int TS>
+__device__ __forceinline__ void mfma_step(float a, float b, float (&c)[TileGeom<TS>::NACC]) {
+    typedef float vacc __attribute__((ext_vector_type(TileGeom<TS>::NACC)));
+    vacc x;
+#pragma unroll
+    for (int e = 0; e < TileGeom<TS>::NACC; ++e) x[e] = c[e];
+    if constexpr (TS == 32)
+        x = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, x, 0, 0, 0);
+    else
+        x = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, x, 0, 0, 0);
+#pragma unroll
+    for (int e = 0; e < TileGeom<TS>::NACC; ++e) c[e] = x[e];
+}
+
+// k a positive multiple of 16: every wave runs one pipelined chain over the
+// (tile, term, slice) items of its tiles.  The operands of the item two ahead
+// are loaded unconditionally (past the end the last item reloads itself) with
+// a fixed instruction sequence, so the wait for an item's operands covers only
+// its own loads.
+// VEC: the operands stored along k are 16-byte aligned (float4 loads).
+template <bool TA, bool TB, int TS, int NW, bool VEC>
+__device__ __forceinline__ void gemm_tiles_full(int m, int n, int k, const float *A0, const float *A1, int lda,
+                                             const float *B0, const float *B1, int ldb, float *C0, int ldc, bool acc,
+                                             int nseg, int nb, long sA, long sB, long sC) {
+    constexpr int KL = TileGeom<TS>::KL, NACC = TileGeom<TS>::NACC;
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int r = lane % TS, h = lane / TS;
+    const int tm = (m + TS - 1) / TS, tn = (n + TS * NW - 1) / (TS * NW), ntile = nb * tm * tn;
+    if (w >= ntile) return;
+    const int ntw = (ntile - w + MQE_WAVES - 1) / MQE_WAVES;  // tiles of this wave
+    const int nsk = k >> 4, nsl = nseg * nsk, nit = ntw * nsl;
+    auto load = [&](int i, float (&a)[KL], float (&b)[NW][KL]) {
+        const int j = i / nsl, rem = i - j * nsl, seg = rem / nsk, sl = rem - seg * nsk;
+        const float *pa, *pb[NW];
+        tile_ptrs<TA, TB, TS, NW>(w + MQE_WAVES * j, seg, m, n, tm, tn, A0, A1, lda, B0, B1, ldb, sA, sB, r, pa, pb);
+        const int kb = 16 * sl + KL * h;
+        load_run<TA, KL>(pa, lda, kb, VEC, a);
+#pragma unroll
+        for (int q = 0; q < NW; ++q) load_run<!TB, KL>(pb[q], ldb, kb, VEC, b[q]);
+    };
+    // three operand sets in rotation (the loop is unrolled by three, so no
+    // register copies wait on loads in flight): the loads run two items ahead
+    // of the MFMAs
+    float a0[KL], b0[NW][KL], a1[KL], b1[NW][KL], a2[KL], b2[NW][KL], c[NW][NACC];
+#pragma unroll
+    for (int q = 0; q < NW; ++q)
+#pragma unroll
+        for (int e = 0; e < NACC; ++e) c[q][e] = 0.f;
+    auto step = [&](int i, const float (&a)[KL], const float (&b)[NW][KL]) {
+#pragma unroll
+        for (int u = 0; u < KL; ++u)
+#pragma unroll
+            for (int q = 0; q < NW; ++q) mfma_step<TS>(a[u], b[q][u], c[q]);
+        if ((i + 1) % nsl == 0) {  // the tile is complete
+            tile_store<TS, NW>(C0, ldc, sC, w + MQE_WAVES * (i / nsl), m, n, tm, tn, r, h, acc, c);
+#pragma unroll
+            for (int q = 0; q < NW; ++q)
+#pragma unroll
+                for (int e = 0; e < NACC; ++e) c[q][e] = 0.f;
+        }
+    };
+    load(0, a0, b0);
+    load(min(1, nit - 1), a1, b1);
+    for (int i = 0; i < nit; i += 3) {
+        load(min(i + 2, nit - 1), a2, b2);
+        __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of the MFMAs
+        step(i, a0, b0);
+        if (i + 1 >= nit) break;
+        load(min(i + 3, nit - 1), a0, b0);
+        __builtin_amdgcn_sched_barrier(0);
+        step(i + 1, a1, b1);
+        if (i + 2 >= nit) break;
+        load(min(i + 4, nit - 1), a1, b1);
+        __builtin_amdgcn_sched_barrier(0);
+        step(i + 2, a2, b2);
+    }
+}
+
+// Any k: per tile and slice, predicated loads (zeros at k >= k), no pipelining.
+template <bool TA, bool TB, int TS, int NW>
+__device__ __forceinline__ void gemm_tiles_pred(int m, int n, int k, const float *A0, const float *A1, int lda,
+                                             const float *B0, const float *B1, int ldb, float *C0, int ldc, bool acc,
+                                             int nseg, int nb, long sA, long sB, long sC) {
+    constexpr int KL = TileGeom<TS>::KL, NACC = TileGeom<TS>::NACC;
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int r = lane % TS, h = lane / TS;
+    const int tm = (m + TS - 1) / TS, tn = (n + TS * NW - 1) / (TS * NW), ntile = nb * tm * tn;
+    for (int tb = w; tb < ntile; tb += MQE_WAVES) {
+        float c[NW][NACC];
+#pragma unroll
+        for (int q = 0; q < NW; ++q)
+#pragma unroll
+            for (int e = 0; e < NACC; ++e) c[q][e] = 0.f;
+        for (int seg = 0; seg < nseg; ++seg) {
+            const float *pa, *pb[NW];
+            tile_ptrs<TA, TB, TS, NW>(tb, seg, m, n, tm, tn, A0, A1, lda, B0, B1, ldb, sA, sB, r, pa, pb);
+            for (int k0 = 0; k0 < k; k0 += 16) {
+                float a[KL], b[NW][KL];
+                load_run_pred<TA, KL>(pa, lda, k0 + KL * h, k, a);
+#pragma unroll
+                for (int q = 0; q < NW; ++q) load_run_pred<!TB, KL>(pb[q], ldb, k0 + KL * h, k, b[q]);
+#pragma unroll
+                for (int u = 0; u < KL; ++u)
+#pragma unroll
+                    for (int q = 0; q < NW; ++q) mfma_step<TS>(a[u], b[q][u], c[q]);
+            }
+        }
+        tile_store<TS, NW>(C0, ldc, sC, tb, m, n, tm, tn, r, h, acc, c);
+    }
+}
+
+template <bool TA, bool TB, int TS, int NW>
+__device__ __forceinline__ void gemm_tiles(int m, int n, int k, const float *A0, const float *A1, int lda,
+                                           const float *B0, const float *B1, int ldb, float *C0, int ldc, bool acc,
+                                           int nseg, int nb, long sA, long sB, long sC) {
+    if (k >= 16 && (k & 15) == 0) {
+        const bool vec = (!TA || aligned16(A0, nseg > 1 ? A1 : A0, lda, sA)) &&
+                         (TB || aligned16(B0, nseg > 1 ? B1 : B0, ldb, sB));
+        if (vec)
+            gemm_tiles_full<TA, TB, TS, NW, true>(m, n, k, A0, A1, lda, B0, B1, ldb, C0, ldc, acc, nseg, nb, sA, sB,
+                                                  sC);
+        else
+            gemm_tiles_full<TA, TB, TS, NW, false>(m, n, k, A0, A1, lda, B0, B1, ldb, C0, ldc, acc, nseg, nb, sA, sB,
+                                                   sC);
+    } else {
+        gemm_tiles_pred<TA, TB, TS, NW>(m, n, k, A0, A1, lda, B0, B1, ldb, C0, ldc, acc, nseg, nb, sA, sB, sC);
+    }
+}
+
+// Single-term products with the original entry points (also used by the
+// proposal network).
+template <bool TA, bool TB>
+__device__ __forceinline__ void gemm_mfma(int m, int n, int k, const float *A0, int lda, const float *B0, int ldb,
+                                          float *C0, int ldc, bool acc, int nb = 1, long sA = 0, long sB = 0,
+                                          long sC = 0) {
+    gemm_tiles<TA, TB, 32, 1>(m, n, k, A0, A0, lda, B0, B0, ldb, C0, ldc, acc, 1, nb, sA, sB, sC);
+}
+
 // Output tile shapes of the FP32 matrix-core products (chosen per walker bond
 // bucket by the engine's tile table).
 enum : int { TILE_16 = 16, TILE_32 = 32, TILE_WIDE = 64 };
+
+// The matrix-core product with the output tile shape `tile`.
+template <bool TA, bool TB>
+__device__ __forceinline__ void gemm_f32(int m, int n, int k, const float *A0, const float *A1, int lda,
+                                         const float *B0, const float *B1, int ldb, float *C, int ldc, bool acc,
+                                         int nseg, int nb, long sA, long sB, long sC, int tile) {
+    if (tile == TILE_16)
+        gemm_tiles<TA, TB, 16, 1>(m, n, k, A0, A1, lda, B0, B1, ldb, C, ldc, acc, nseg, nb, sA, sB, sC);
+    else if (tile == TILE_WIDE && n > 32)
+        gemm_tiles<TA, TB, 32, 2>(m, n, k, A0, A1, lda, B0, B1, ldb, C, ldc, acc, nseg, nb, sA, sB, sC);
+    else
+        gemm_tiles<TA, TB, 32, 1>(m, n, k, A0, A1, lda, B0, B1, ldb, C, ldc, acc, nseg, nb, sA, sB, sC);
+}
+
+// Vector-unit product (FP64, and FP32 products too small for the matrix cores).
+template <bool TA, bool TB, typename T>
+__device__ __forceinline__ void gemm_valu(int m, int n, int k, const T *A0, const T *A1, int lda, const T *B0,
+                                          const T *B1, int ldb, T *C, int ldc, bool acc, int nseg, int nb, long sA,
+                                          long sB, long sC) {
+    const int mn = m * n;
+    for (int e = threadIdx.x; e < nb * mn; e += MQE_NT) {
+        const int bt = e / mn, x = e - bt * mn;
+        const int r = x % m, c = x / m;
+        T *Cb = C + (size_t)bt * sC;
+        T s = acc ? Cb[r + (size_t)ldc * c] : T(0);
+        for (int sg = 0; sg < nseg; ++sg) {
+            const T *Ab = (sg ? A1 : A0) + (size_t)bt * sA, *Bb = (sg ? B1 : B0) + (size_t)bt * sB;
+            const T *ap = TA ? (Ab + (size_t)lda * r) : (Ab + r);
+            const T *bp = TB ? (Bb + c) : (Bb + (size_t)ldb * c);
+            for (int q = 0; q < k; ++q) {
+                const T av = TA ? ap[q] : ap[(size_t)lda * q];
+                const T bv = TB ? bp[(size_t)ldb * q] : bp[q];
+                s += av * bv;
+            }
+        }
+        Cb[r + (size_t)ldc * c] = s;
+    }
+}
 
 // ---------------------------------------------------------------------------
 // nb products of one shape, C_b (+)= op(A_b) op(B_b) with A_b = A + b sA (and so
@@ -372,31 +438,28 @@ __device__ __forceinline__ void gemm_batch(int nb, int m, int n, int k, const T 
                                            int ldb, long sB, T *C, int ldc, long sC, bool acc, int tile = TILE_32) {
     if constexpr (sizeof(T) == sizeof(float)) {
         if (m >= 8 && n >= 8 && k >= 4) {
-            if (tile == TILE_16)
-                gemm_mfma16<TA, TB>(m, n, k, (const float *)A, lda, (const float *)B, ldb, (float *)C, ldc, acc, nb,
-                                    sA, sB, sC);
-            else
-                gemm_mfma<TA, TB>(m, n, k, (const float *)A, lda, (const float *)B, ldb, (float *)C, ldc, acc, nb, sA,
-                                  sB, sC);
+            gemm_f32<TA, TB>(m, n, k, (const float *)A, (const float *)A, lda, (const float *)B, (const float *)B, ldb,
+                             (float *)C, ldc, acc, 1, nb, sA, sB, sC, tile);
             return;
         }
     }
-    const int mn = m * n;
-    for (int e = threadIdx.x; e < nb * mn; e += MQE_NT) {
-        const int bt = e / mn, x = e - bt * mn;
-        const int r = x % m, c = x / m;
-        const T *Ab = A + (size_t)bt * sA, *Bb = B + (size_t)bt * sB;
-        T *Cb = C + (size_t)bt * sC;
-        T s = acc ? Cb[r + ldc * c] : T(0);
-        const T *ap = TA ? (Ab + (size_t)lda * r) : (Ab + r);
-        const T *bp = TB ? (Bb + c) : (Bb + (size_t)ldb * c);
-        for (int q = 0; q < k; ++q) {
-            const T a = TA ? ap[q] : ap[(size_t)lda * q];
-            const T b = TB ? bp[(size_t)ldb * q] : bp[q];
-            s += a * b;
+    gemm_valu<TA, TB, T>(m, n, k, A, A, lda, B, B, ldb, C, ldc, acc, 1, nb, sA, sB, sC);
+}
+
+// Two-term product C = op(A0) op(B0) + op(A1) op(B1) (the sums over a physical
+// index): one accumulation chain per output tile.
+template <bool TA, bool TB, typename T>
+__device__ __forceinline__ void gemm2(int m, int n, int k, const T *A0, const T *A1, int lda, const T *B0,
+                                      const T *B1, int ldb, T *C, int ldc, int tile = TILE_32, int nb = 1, long sA = 0,
+                                      long sB = 0, long sC = 0) {
+    if constexpr (sizeof(T) == sizeof(float)) {
+        if (m >= 8 && n >= 8 && k >= 4) {
+            gemm_f32<TA, TB>(m, n, k, (const float *)A0, (const float *)A1, lda, (const float *)B0, (const float *)B1,
+                             ldb, (float *)C, ldc, false, 2, nb, sA, sB, sC, tile);
+            return;
         }
-        Cb[r + ldc * c] = s;
     }
+    gemm_valu<TA, TB, T>(m, n, k, A0, A1, lda, B0, B1, ldb, C, ldc, false, 2, nb, sA, sB, sC);
 }
 
 // ---------------------------------------------------------------------------
@@ -407,30 +470,7 @@ __device__ __forceinline__ void gemm_batch(int nb, int m, int n, int k, const T 
 template <bool TA, bool TB, typename T>
 __device__ __forceinline__ void gemm(int m, int n, int k, const T *A, int lda, const T *B, int ldb, T *C, int ldc,
                                      bool acc, int tile = TILE_32) {
-    if constexpr (sizeof(T) == sizeof(float)) {
-        if (m >= 8 && n >= 8 && k >= 4) {
-            if (tile == TILE_16)
-                gemm_mfma16<TA, TB>(m, n, k, (const float *)A, lda, (const float *)B, ldb, (float *)C, ldc, acc);
-            else if (tile == TILE_WIDE && n > 32)
-                gemm_mfma_wide<TA, TB>(m, n, k, (const float *)A, lda, (const float *)B, ldb, (float *)C, ldc, acc);
-            else
-                gemm_mfma<TA, TB>(m, n, k, (const float *)A, lda, (const float *)B, ldb, (float *)C, ldc, acc);
-            return;
-        }
-    }
-    const int mn = m * n;
-    for (int e = threadIdx.x; e < mn; e += MQE_NT) {
-        const int r = e % m, c = e / m;
-        T s = acc ? C[r + ldc * c] : T(0);
-        const T *ap = TA ? (A + (size_t)lda * r) : (A + r);
-        const T *bp = TB ? (B + c) : (B + (size_t)ldb * c);
-        for (int q = 0; q < k; ++q) {
-            const T a = TA ? ap[q] : ap[(size_t)lda * q];
-            const T b = TB ? bp[(size_t)ldb * q] : bp[q];
-            s += a * b;
-        }
-        C[r + ldc * c] = s;
-    }
+    gemm_batch<TA, TB, T>(1, m, n, k, A, lda, 0, B, ldb, 0, C, ldc, 0, acc, tile);
 }
 
 // ---------------------------------------------------------------------------

@@ -198,7 +198,7 @@ struct Ctx {
 // Y = L_i phi_i (t_i x 2 w_{i+1});  G = T_i^T Y (2 t_{i+1} x 2 w_{i+1}), block
 // (s', s) = T_i[s']^T L_i phi_i[s] at G + s' t + s w (2t), ld 2t.
 template <typename T>
-__device__ void compute_G(const Ctx<T> &C, int i, const T *Lenv_i) {
+__device__ __forceinline__ void compute_G(const Ctx<T> &C, int i, const T *Lenv_i) {
     const int ti = C.tv[i], tn = C.tv[i + 1], wi = C.wv[i], wn = C.wv[i + 1];
     T *Y = C.ts + C.sl.Y, *G = C.ts + C.sl.G;
     gemm<false, false, T>(ti, 2 * wn, wi, Lenv_i, ti, C.site(i), wi, Y, ti, false, C.tile);
@@ -210,7 +210,7 @@ __device__ void compute_G(const Ctx<T> &C, int i, const T *Lenv_i) {
 // W[t] = R_{j+1} phi_j[t]^T (t_{j+1} x w_j);  V[t'] = T_j[t'] W (t_j x 2 w_j);
 // block (t', t) at V + t' (t_j 2 w_j) + t (t_j w_j), ld t_j.
 template <typename T>
-__device__ void compute_V(const Ctx<T> &C, int j) {
+__device__ __forceinline__ void compute_V(const Ctx<T> &C, int j) {
     const int tj = C.tv[j], tn = C.tv[j + 1], wj = C.wv[j], wn = C.wv[j + 1];
     T *W = C.ts + C.sl.W, *V = C.ts + C.sl.V;
     const T *R = C.Rk(j + 1), *ph = C.site(j);
@@ -225,26 +225,23 @@ __device__ void compute_V(const Ctx<T> &C, int j) {
 // Carry four channels (t_k x w_k, ld t_k, contiguous at src) through site k
 // into dst (t_{k+1} x w_{k+1}).  Uses Y as temporary.
 template <typename T>
-__device__ void carry_channels(const Ctx<T> &C, int k, const T *src, T *dst) {
+__device__ __forceinline__ void carry_channels(const Ctx<T> &C, int k, const T *src, T *dst) {
     const int tk = C.tv[k], tn = C.tv[k + 1], wk = C.wv[k], wn = C.wv[k + 1];
     T *Y = C.ts + C.sl.Y4;  // Y_ch = src_ch phi_k (tk x 2 wn), the four channels one after the other
     const T *Tk = C.tsite(k);
     const long sy = (long)tk * 2 * wn;
     gemm_batch<false, false, T>(4, tk, 2 * wn, wk, src, tk, (long)tk * wk, C.site(k), wk, 0, Y, tk, sy, false, C.tile);
     __syncthreads();
-    // dst_ch = T_k[0]^T Y_ch[0] + T_k[1]^T Y_ch[1]; no barrier between the two:
-    // the accumulating product has the same shape, so every thread re-reads
-    // only the output elements it wrote itself
-    gemm_batch<true, false, T>(4, tn, wn, tk, Tk, tk, 0, Y, tk, sy, dst, tn, (long)tn * wn, false, C.tile);
-    gemm_batch<true, false, T>(4, tn, wn, tk, Tk + (size_t)tk * tn, tk, 0, Y + (size_t)tk * wn, tk, sy, dst, tn,
-                               (long)tn * wn, true, C.tile);
+    // dst_ch = T_k[0]^T Y_ch[0] + T_k[1]^T Y_ch[1], one accumulation chain per output tile
+    gemm2<true, false, T>(tn, wn, tk, Tk, Tk + (size_t)tk * tn, tk, Y, Y + (size_t)tk * wn, tk, dst, tn, C.tile, 4, 0,
+                          sy, (long)tn * wn);
     __syncthreads();
 }
 
 // Two-site mixed tensor M[(s'+2t')*4 + (s+2t)] = <G(s',s), V(t',t)> at a bond
 // with trial dim tb and walker dim wb.  g[ch], ch = 2 s' + s, with leading dim gld.
 template <typename T>
-__device__ void pair_tensor(const Ctx<T> &C, int tb, int wb, const T *const g[4], int gld, double *M) {
+__device__ __forceinline__ void pair_tensor(const Ctx<T> &C, int tb, int wb, const T *const g[4], int gld, double *M) {
     const T *V = C.ts + C.sl.V;
     double acc[16];
 #pragma unroll
@@ -279,7 +276,7 @@ __device__ void pair_tensor(const Ctx<T> &C, int tb, int wb, const T *const g[4]
 
 // Mixed tensor for the pair (i, j) from L_i, sites i..j and R_{j+1}.
 template <typename T>
-__device__ void pair_for_coupling(const Ctx<T> &C, int i, int j, double *M) {
+__device__ __forceinline__ void pair_for_coupling(const Ctx<T> &C, int i, int j, double *M) {
     compute_G(C, i, C.Lk(i));
     const int tn = C.tv[i + 1], wn = C.wv[i + 1];
     const T *G = C.ts + C.sl.G;
@@ -314,36 +311,32 @@ __device__ void pair_for_coupling(const Ctx<T> &C, int i, int j, double *M) {
 
 // L_{k+1} = sum_s T_k[s]^T L_k phi_k[s]
 template <typename T>
-__device__ void advance_L(const Ctx<T> &C, int k) {
+__device__ __forceinline__ void advance_L(const Ctx<T> &C, int k) {
     const int tk = C.tv[k], tn = C.tv[k + 1], wk = C.wv[k], wn = C.wv[k + 1];
     T *Y = C.ts + C.sl.Y;
     gemm<false, false, T>(tk, 2 * wn, wk, C.Lk(k), tk, C.site(k), wk, Y, tk, false, C.tile);
     __syncthreads();
     const T *Tk = C.tsite(k);
     T *Ln = C.Lk(k + 1);
-    gemm<true, false, T>(tn, wn, tk, Tk, tk, Y, tk, Ln, tn, false, C.tile);
-    // same-shape accumulation: no barrier needed (each thread owns its outputs)
-    gemm<true, false, T>(tn, wn, tk, Tk + (size_t)tk * tn, tk, Y + (size_t)tk * wn, tk, Ln, tn, true, C.tile);
+    gemm2<true, false, T>(tn, wn, tk, Tk, Tk + (size_t)tk * tn, tk, Y, Y + (size_t)tk * wn, tk, Ln, tn, C.tile);
     __syncthreads();
 }
 
 // Nn = sum_s phi_k[s]^T N phi_k[s]  (w_{k+1} x w_{k+1})
 template <typename T>
-__device__ void advance_N(const Ctx<T> &C, int k, const T *N, T *Nn) {
+__device__ __forceinline__ void advance_N(const Ctx<T> &C, int k, const T *N, T *Nn) {
     const int wk = C.wv[k], wn = C.wv[k + 1];
     T *X = C.ts + C.sl.Y;
     const T *ph = C.site(k);
     gemm<false, false, T>(wk, 2 * wn, wk, N, wk, ph, wk, X, wk, false, C.tile);
     __syncthreads();
-    gemm<true, false, T>(wn, wn, wk, ph, wk, X, wk, Nn, wn, false, C.tile);
-    // same-shape accumulation: no barrier needed (each thread owns its outputs)
-    gemm<true, false, T>(wn, wn, wk, ph + (size_t)wk * wn, wk, X + (size_t)wk * wn, wk, Nn, wn, true, C.tile);
+    gemm2<true, false, T>(wn, wn, wk, ph, ph + (size_t)wk * wn, wk, X, X + (size_t)wk * wn, wk, Nn, wn, C.tile);
     __syncthreads();
 }
 
 // <phi_k, N phi_k> with the sites right of k right-orthonormal = squared norm.
 template <typename T>
-__device__ double norm2_at(const Ctx<T> &C, int k, const T *N) {
+__device__ __forceinline__ double norm2_at(const Ctx<T> &C, int k, const T *N) {
     const int wk = C.wv[k], wn = C.wv[k + 1];
     T *Z = C.ts + C.sl.Y;
     const T *ph = C.site(k);
@@ -357,23 +350,21 @@ __device__ double norm2_at(const Ctx<T> &C, int k, const T *N) {
 // Right norm environment M_k = sum_s phi_k[s] M_{k+1} phi_k[s]^T (w_k x w_k):
 // the norm of the walker's part right of bond k in the current gauge.
 template <typename T>
-__device__ void advance_M(const Ctx<T> &C, int k) {
+__device__ __forceinline__ void advance_M(const Ctx<T> &C, int k) {
     const int wk = C.wv[k], wn = C.wv[k + 1];
     T *X = C.ts + C.sl.Y;
     const T *ph = C.site(k), *Mn = C.Mk(k + 1);
     T *Mk = C.Mk(k);
     gemm_batch<false, false, T>(2, wk, wn, wn, ph, wk, (long)wk * wn, Mn, wn, 0, X, wk, (long)wk * wn, false, C.tile);
     __syncthreads();
-    gemm<false, true, T>(wk, wk, wn, X, wk, ph, wk, Mk, wk, false, C.tile);
-    // same-shape accumulation: no barrier needed (each thread owns its outputs)
-    gemm<false, true, T>(wk, wk, wn, X + (size_t)wk * wn, wk, ph + (size_t)wk * wn, wk, Mk, wk, true, C.tile);
+    gemm2<false, true, T>(wk, wk, wn, X, X + (size_t)wk * wn, wk, ph, ph + (size_t)wk * wn, wk, Mk, wk, C.tile);
     __syncthreads();
 }
 
 // Squared norm with the left norm environment N (bond k) and the right norm
 // environment M_{k+1}: sum_s tr(phi_k[s]^T N phi_k[s] M_{k+1}).
 template <typename T>
-__device__ double norm2_at_M(const Ctx<T> &C, int k, const T *N) {
+__device__ __forceinline__ double norm2_at_M(const Ctx<T> &C, int k, const T *N) {
     const int wk = C.wv[k], wn = C.wv[k + 1];
     T *Z = C.ts + C.sl.Y, *Z2 = C.ts + C.sl.TMP;
     const T *ph = C.site(k), *Mn = C.Mk(k + 1);
@@ -387,7 +378,7 @@ __device__ double norm2_at_M(const Ctx<T> &C, int k, const T *N) {
 }
 
 template <typename T>
-__device__ void scale_site(const Ctx<T> &C, int k, double f) {
+__device__ __forceinline__ void scale_site(const Ctx<T> &C, int k, double f) {
     T *ph = C.site(k);
     const int n = C.wv[k] * C.wv[k + 1] * 2;
     for (int e = threadIdx.x; e < n; e += MQE_NT) ph[e] = (T)((double)ph[e] * f);
@@ -396,7 +387,7 @@ __device__ void scale_site(const Ctx<T> &C, int k, double f) {
 
 // phi_k[s] <- sum_s0 O[s + 2 s0] phi_k[s0]
 template <typename T>
-__device__ void apply_op(const Ctx<T> &C, int k, const double *O) {
+__device__ __forceinline__ void apply_op(const Ctx<T> &C, int k, const double *O) {
     T *ph = C.site(k);
     const int blk = C.wv[k] * C.wv[k + 1];
     const T o00 = (T)O[0], o10 = (T)O[1], o01 = (T)O[2], o11 = (T)O[3];
@@ -412,7 +403,7 @@ __device__ void apply_op(const Ctx<T> &C, int k, const double *O) {
 // phi_{k-1} <- phi_{k-1} Lmat, then R_k = sum_t T_k[t] R_{k+1} Q[t]^T.
 // Leaves V (all four blocks) computed for bond k.
 template <typename T>
-__device__ void lq_site(const Ctx<T> &C, int k) {
+__device__ __forceinline__ void lq_site(const Ctx<T> &C, int k) {
     const int wk = C.wv[k], wn = C.wv[k + 1], wp = C.wv[k - 1];
     const int m = 2 * wn, n = wk;
     // the FP64 factorisation's sequential column steps are latency-bound: the
@@ -459,7 +450,7 @@ __device__ void lq_site(const Ctx<T> &C, int k) {
 
 // <trial|phi> and sum_s sz[s] <trial|S^z_0|phi> at site 0 (L_0 = 1, right part via R_1).
 template <typename T>
-__device__ void site0_overlap(const Ctx<T> &C, double *ov, double *szv) {
+__device__ __forceinline__ void site0_overlap(const Ctx<T> &C, double *ov, double *szv) {
     const int t1 = C.tv[1], w1 = C.wv[1];
     const T *T0 = C.tsite(0), *p0 = C.site(0), *R1 = C.Rk(1);
     double a[2] = {0.0, 0.0};
@@ -474,7 +465,7 @@ __device__ void site0_overlap(const Ctx<T> &C, double *ov, double *szv) {
 }
 
 template <typename T>
-__device__ void set_one(T *p) {
+__device__ __forceinline__ void set_one(T *p) {
     if (threadIdx.x == 0) p[0] = (T)1;
     __syncthreads();
 }
@@ -482,7 +473,7 @@ __device__ void set_one(T *p) {
 // Right-canonicalise the whole walker with R environments, normalise at site 0.
 // Returns the (sign-fixed) overlap of the normalised state.
 template <typename T>
-__device__ double right_canonicalize(const Ctx<T> &C) {
+__device__ __forceinline__ double right_canonicalize(const Ctx<T> &C) {
     const int L = C.S->L;
     set_one(C.Rk(L));
     set_one(C.Mk(L));
@@ -508,7 +499,7 @@ __device__ double right_canonicalize(const Ctx<T> &C) {
 // Returns e_tot = <trial|H|phi> of the state as it was on entry and writes the
 // overlap of the normalised state to *O_out, <trial|phi> before normalisation to *Ou.
 template <typename T>
-__device__ double energy_sweep(const Ctx<T> &C, double *O_out, double *Ou, bool canon) {
+__device__ __forceinline__ double energy_sweep(const Ctx<T> &C, double *O_out, double *Ou, bool canon) {
     const Sys &S = *C.S;
     const int L = S.L;
     __shared__ double M[16];
@@ -617,7 +608,7 @@ __device__ double energy_sweep(const Ctx<T> &C, double *O_out, double *Ou, bool 
 }
 
 template <typename T>
-__device__ void apply_field_everywhere(const Ctx<T> &C) {
+__device__ __forceinline__ void apply_field_everywhere(const Ctx<T> &C) {
     const double O[4] = {C.S->ssp[0], 0.0, 0.0, C.S->ssp[1]};
     for (int k = 0; k < C.S->L; ++k) apply_op(C, k, O);
 }
