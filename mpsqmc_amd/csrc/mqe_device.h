@@ -362,15 +362,12 @@ template <bool TA, bool TB, int TS, int NW>
 __device__ __forceinline__ void gemm_tiles(int m, int n, int k, const float *A0, const float *A1, int lda,
                                            const float *B0, const float *B1, int ldb, float *C0, int ldc, bool acc,
                                            int nseg, int nb, long sA, long sB, long sC) {
-    if (k >= 16 && (k & 15) == 0) {
-        const bool vec = (!TA || aligned16(A0, nseg > 1 ? A1 : A0, lda, sA)) &&
-                         (TB || aligned16(B0, nseg > 1 ? B1 : B0, ldb, sB));
-        if (vec)
-            gemm_tiles_full<TA, TB, TS, NW, true>(m, n, k, A0, A1, lda, B0, B1, ldb, C0, ldc, acc, nseg, nb, sA, sB,
-                                                  sC);
-        else
-            gemm_tiles_full<TA, TB, TS, NW, false>(m, n, k, A0, A1, lda, B0, B1, ldb, C0, ldc, acc, nseg, nb, sA, sB,
-                                                   sC);
+    // the pipelined path needs whole slices and 16-byte aligned operands stored
+    // along k (bond dimensions that are multiples of 16 give both)
+    const bool vec = (!TA || aligned16(A0, nseg > 1 ? A1 : A0, lda, sA)) &&
+                     (TB || aligned16(B0, nseg > 1 ? B1 : B0, ldb, sB));
+    if (k >= 16 && (k & 15) == 0 && vec) {
+        gemm_tiles_full<TA, TB, TS, NW, true>(m, n, k, A0, A1, lda, B0, B1, ldb, C0, ldc, acc, nseg, nb, sA, sB, sC);
     } else {
         gemm_tiles_pred<TA, TB, TS, NW>(m, n, k, A0, A1, lda, B0, B1, ldb, C0, ldc, acc, nseg, nb, sA, sB, sC);
     }
