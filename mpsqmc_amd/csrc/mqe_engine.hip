@@ -11,9 +11,13 @@
 //     M   = <G, V>                                (16 numbers = all two-site
 //                                                   mixed matrix elements)
 //     combos, PDF over the grid, CDF sample, weight update (MPSQMC2.cpp:540-588)
-//     apply the sampled operator on sites i and j (MPSstate.cpp:575-633)
-//     normalise: left norm environment N (nearest-neighbour fast path) or an
-//     FP64 Householder LQ sweep over j..i+1 (long-range), overlap from M.
+//     apply the sampled operator on sites i and j (MPSstate.cpp:575-633), the
+//     right one scaled by O / <trial|B(x) phi> so that the overlap stays O and
+//     the norm stays near 1; the exact norm is taken once, in the energy
+//     sweep (every quantity of the step is a ratio in which the walker's
+//     scale cancels);
+//     right environments R_k of sites a long-range coupling changed are
+//     rebuilt when a later coupling needs them.
 //   energy sweep (right -> left):
 //     FP64 Householder LQ per site (right-canonical form for the next step,
 //     right environments R_k), bond energies <trial|S_i.S_j|phi> from the
@@ -130,7 +134,7 @@ struct Pool {
 
 // Scratch layout of one workgroup (sizes in elements).
 struct ScratchLayout {
-    long Y, Y4, G, GcA, GcB, W, V, N0, N1, TMP, LM, t_total;  // T region
+    long Y, Y4, G, GcA, GcB, W, V, TMP, LM, t_total;  // T region
     long AQ, RQ, TAU, PARTS, DOTS, PDF, CUM, d_total;     // double region
 };
 
@@ -145,8 +149,6 @@ __host__ __device__ inline ScratchLayout scratch_layout(int Dt, int Dw, int Ng) 
     s.GcB = o; o += 4L * Dt * Dw;
     s.W = o; o += 2L * Dt * Dw;
     s.V = o; o += 4L * Dt * Dw;
-    s.N0 = o; o += (long)Dw * Dw;
-    s.N1 = o; o += (long)Dw * Dw;
     s.TMP = o; o += 2L * Dw * Dw;
     s.LM = o; o += (long)Dw * Dw;
     s.t_total = o;
@@ -322,29 +324,15 @@ __device__ __forceinline__ void advance_L(const Ctx<T> &C, int k) {
     __syncthreads();
 }
 
-// Nn = sum_s phi_k[s]^T N phi_k[s]  (w_{k+1} x w_{k+1})
+// R_k = sum_t T_k[t] R_{k+1} phi_k[t]^T (the diagonal blocks of V at bond k)
 template <typename T>
-__device__ __forceinline__ void advance_N(const Ctx<T> &C, int k, const T *N, T *Nn) {
-    const int wk = C.wv[k], wn = C.wv[k + 1];
-    T *X = C.ts + C.sl.Y;
-    const T *ph = C.site(k);
-    gemm<false, false, T>(wk, 2 * wn, wk, N, wk, ph, wk, X, wk, false, C.tile);
+__device__ __forceinline__ void rebuild_R(const Ctx<T> &C, int k) {
+    compute_V(C, k);
+    const int tk = C.tv[k], wk = C.wv[k];
+    const T *V = C.ts + C.sl.V;
+    T *Rk = C.Rk(k);
+    for (int e = threadIdx.x; e < tk * wk; e += MQE_NT) Rk[e] = V[e] + V[(size_t)tk * 2 * wk + (size_t)tk * wk + e];
     __syncthreads();
-    gemm2<true, false, T>(wn, wn, wk, ph, ph + (size_t)wk * wn, wk, X, X + (size_t)wk * wn, wk, Nn, wn, C.tile);
-    __syncthreads();
-}
-
-// <phi_k, N phi_k> with the sites right of k right-orthonormal = squared norm.
-template <typename T>
-__device__ __forceinline__ double norm2_at(const Ctx<T> &C, int k, const T *N) {
-    const int wk = C.wv[k], wn = C.wv[k + 1];
-    T *Z = C.ts + C.sl.Y;
-    const T *ph = C.site(k);
-    gemm<false, false, T>(wk, 2 * wn, wk, N, wk, ph, wk, Z, wk, false, C.tile);
-    __syncthreads();
-    double s = 0.0;
-    for (int e = threadIdx.x; e < wk * 2 * wn; e += MQE_NT) s += (double)ph[e] * (double)Z[e];
-    return block_sum(s, C.red);
 }
 
 // Right norm environment M_k = sum_s phi_k[s] M_{k+1} phi_k[s]^T (w_k x w_k):
@@ -443,8 +431,6 @@ __device__ __forceinline__ void lq_site(const Ctx<T> &C, int k) {
     const T *V = C.ts + C.sl.V;
     T *Rk = C.Rk(k);
     for (int e = threadIdx.x; e < tk * wk; e += MQE_NT) Rk[e] = V[e] + V[(size_t)tk * 2 * wk + (size_t)tk * wk + e];
-    T *Mk = C.Mk(k);  // sites >= k right-orthonormal
-    for (int e = threadIdx.x; e < wk * wk; e += MQE_NT) Mk[e] = (e % wk == e / wk) ? (T)1 : (T)0;
     __syncthreads();
 }
 
@@ -476,7 +462,6 @@ template <typename T>
 __device__ __forceinline__ double right_canonicalize(const Ctx<T> &C) {
     const int L = C.S->L;
     set_one(C.Rk(L));
-    set_one(C.Mk(L));
     for (int k = L - 1; k >= 1; --k) lq_site(C, k);
     double ov, szv;
     site0_overlap(C, &ov, &szv);
@@ -686,17 +671,18 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker_kernel(Sys S, Pool P, const 
         apply_field_everywhere(C);
         O = right_canonicalize(C);
     }
-    T *Ncur = C.ts + C.sl.N0, *Nnxt = C.ts + C.sl.N1;
-    set_one(Ncur);
-    int p = 0;
+    // The walker is not normalised between couplings: the sampled operator of
+    // the right site is scaled so that the overlap stays O (see the file
+    // header), and the energy sweep takes the norm.  p: left environments L_k
+    // are valid for k <= p; rv: right environments R_k for k >= rv.
+    int p = 0, rv = 1;
     for (int c = 0; c < S.nC; ++c) {
         const int i = S.first[c], j = S.second[c];
         while (p < i) {
             advance_L(C, p);
-            advance_N(C, p, Ncur, Nnxt);
-            T *t = Ncur; Ncur = Nnxt; Nnxt = t;
             ++p;
         }
+        while (rv > j + 1) rebuild_R(C, --rv);
         pair_for_coupling(C, i, j, M);
         // combos[a + 4b] = sqrt(s_a s_b) <trial| U_a (x) V_b |phi>  (MPSQMC2.cpp:540-549)
         const double *tS = S.tS + 4 * c, *tU = S.tU + 16 * c, *tVT = S.tVT + 16 * c;
@@ -762,13 +748,14 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker_kernel(Sys S, Pool P, const 
             }
         }
         __syncthreads();
-        const double fstar = sh_scal[0];
+        // right site scaled by O / <trial|B(x) phi> = O / (4 f*): the overlap stays O
+        const double alpha = O / (4.0 * sh_scal[0]);
+        const double Va[4] = {alpha * Vx[0], alpha * Vx[1], alpha * Vx[2], alpha * Vx[3]};
         apply_op(C, i, Ux);
-        apply_op(C, j, Vx);
-        const int nf = S.nextfirst[c];
-        double nrm2;
-        if (j == i + 1 && nf > i) {
-            // nearest-neighbour fast path: boundary moves to j
+        apply_op(C, j, Va);
+        if (j == i + 1 && S.nextfirst[c] > i) {
+            // nearest neighbours, and the next coupling starts right of i: the
+            // left boundary moves to j, L_{i+1} = sum_{s,s0} Ux[s,s0] G(s, s0)
             const int tn = tv[i + 1], wn = wv[i + 1];
             const T *G = C.ts + C.sl.G;
             T *Ln = C.Lk(i + 1);
@@ -776,54 +763,24 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker_kernel(Sys S, Pool P, const 
             for (int e = threadIdx.x; e < tn * wn; e += MQE_NT) {
                 const int a = e % tn, bq = e / tn;
                 const size_t o = a + (size_t)2 * tn * bq;
-                // L_{i+1} = sum_{s,s0} Ux[s,s0] G(s, s0)
                 const T g00 = G[o], g01 = G[o + (size_t)wn * 2 * tn], g10 = G[o + tn],
                         g11 = G[o + tn + (size_t)wn * 2 * tn];
                 Ln[a + (size_t)tn * bq] = u00 * g00 + u01 * g01 + u10 * g10 + u11 * g11;
             }
             __syncthreads();
-            advance_N(C, i, Ncur, Nnxt);
-            T *t = Ncur; Ncur = Nnxt; Nnxt = t;
             p = i + 1;
-            nrm2 = norm2_at_M(C, j, Ncur);
-            const double nrm = sqrt(nrm2);
-            scale_site(C, j, 1.0 / nrm);
-            O = 4.0 * fstar / nrm;
         } else {
-            // sites i+1..j are left as they are (no re-orthonormalisation): their
-            // right environments and right norm environments are rebuilt, and the
-            // norm comes from the left norm environment, site i and M_{i+1}
-            for (int k = j; k >= i + 1; --k) {
-                compute_V(C, k);
-                const int tk = tv[k], wk = wv[k];
-                const T *V = C.ts + C.sl.V;
-                T *Rk = C.Rk(k);
-                for (int e = threadIdx.x; e < tk * wk; e += MQE_NT)
-                    Rk[e] = V[e] + V[(size_t)tk * 2 * wk + (size_t)tk * wk + e];
-                advance_M(C, k);
-            }
-            nrm2 = norm2_at_M(C, i, Ncur);
-            const double nrm = sqrt(nrm2);
-            scale_site(C, i, 1.0 / nrm);
-            O = 4.0 * fstar / nrm;
+            p = i;  // site i changed: L_{i+1} is rebuilt when needed
         }
-    }
-    while (p < L - 1) {
-        advance_L(C, p);
-        advance_N(C, p, Ncur, Nnxt);
-        T *t = Ncur; Ncur = Nnxt; Nnxt = t;
-        ++p;
+        rv = max(rv, j + 1);  // site j changed
     }
     if (S.h != 0.0) {
         apply_field_everywhere(C);
-        set_one(Ncur);
-        for (int k = 0; k < L - 1; ++k) {
-            advance_L(C, k);
-            advance_N(C, k, Ncur, Nnxt);
-            T *t = Ncur; Ncur = Nnxt; Nnxt = t;
-        }
-        const double nrm = sqrt(norm2_at(C, L - 1, Ncur));
-        scale_site(C, L - 1, 1.0 / nrm);
+        p = 0;
+    }
+    while (p < L - 1) {
+        advance_L(C, p);
+        ++p;
     }
     // ---------------- local energy (Walker.cpp:72-77) ----------------
     double On, Ou;
@@ -1802,7 +1759,7 @@ int mqe_create(const mqe_params *p, const mqe_mps *trial, int n_couplings, const
                     n_grid <= t32::MAXG_T32 && !(no32 && no32[0] == '1');
         if (e->tile32) {
             e->wflat = 2L * L * 1024;
-            e->eflat = 2L * (L + 1) * 1024;  // R_k blocks, then the right norm environments M_k
+            e->eflat = (L + 1) * 1024L;  // L_k / R_k blocks
         }
     }
 
@@ -2013,9 +1970,9 @@ void mqe_destroy(mqe_engine *e) {
 
 }  // extern "C"
 
-// Tile path: the boundary environments L_0 = 1, R_L = 1 and M_L = 1 of every
-// slot (the kernel reads them from global memory and never writes them; walker
-// copies carry the right environments but not the left ones), the rest zero.
+// Tile path: the boundary environments L_0 = 1 and R_L = 1 of every slot (the
+// kernel reads them from global memory and never writes them; walker copies
+// carry the right environments but not the left ones), the rest zero.
 static int init_tile_env(mqe_engine *e) {
     if (!e->tile32) return MQE_OK;
     const int L = e->L;
@@ -2023,7 +1980,6 @@ static int init_tile_env(mqe_engine *e) {
     for (int sl = 0; sl < e->p.capacity; ++sl) {
         env[(size_t)sl * e->eflat] = 1.f;
         env[(size_t)sl * e->eflat + (size_t)L * 1024] = 1.f;
-        env[(size_t)sl * e->eflat + (size_t)(2 * L + 1) * 1024] = 1.f;  // M_L = 1
     }
     HIPCHK(hipMemcpy(e->d_Lenv, env.data(), sizeof(float) * env.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(e->d_Renv, env.data(), sizeof(float) * env.size(), hipMemcpyHostToDevice));

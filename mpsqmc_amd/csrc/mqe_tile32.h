@@ -26,18 +26,20 @@ constexpr int LD = 33;           // LDS leading dimension
 constexpr int BLK = 32 * LD;     // floats per LDS block
 constexpr int GB = 1024;         // floats per global block
 
-// LDS map (in LDS blocks).  The center/next walker tensors alternate between
-// blocks 0-1 and 2-3.  During the LQ the FP64 workspace overlays blocks 7-15.
+// LDS map (in LDS blocks).  The walker's center tensors rotate over the block
+// pairs 0-1, 2-3 and 6-7 in the propagation and alternate between 0-1 and 2-3
+// in the energy sweep.  During the LQ the FP64 workspace overlays blocks 7-15.
 enum : int {
     B_BUF0 = 0,  // 2 blocks
     B_BUF1 = 2,  // 2 blocks
     B_L = 4,     // left environment
     B_R = 5,     // right environment
-    B_LM = 6,    // Lmat of the LQ
-    B_N = 7,     // left norm environment
-    B_Y = 8,     // 2 blocks
+    B_LM = 6,    // energy sweep: Lmat of the LQ or the right norm environment M_k
+    B_N = 7,     // energy sweep: LQ workspace
+    B_BUF2 = 6,  // propagation: third center pair (blocks 6-7)
+    B_Y = 8,     // 2 blocks (G: 4 blocks)
     B_W = 10,    // 2 blocks
-    B_V = 12,    // 4 blocks (X after the pair tensor)
+    B_V = 12,    // 4 blocks
     NBLK = 16
 };
 constexpr int MAXG_T32 = 256;                    // grid points: four per lane of one wave
@@ -118,28 +120,6 @@ __device__ __forceinline__ void mma_lr(v16f &acc, const float *A, int lda, const
         const int k = 2 * q + h;
         const float a = TA ? A[k + lda * r] : A[r + lda * k];
         acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b[q], acc, 0, 0, 0);
-    }
-}
-
-// Fragments of X = c0 X0 + c1 X1 (two LDS blocks, leading dim LD) formed on
-// the fly.  A fragment: a[q] = X(r, 2q + h); B fragment: b[q] = X(2q + h, j);
-// T: the blocks hold X transposed.
-template <bool T>
-__device__ __forceinline__ void afrag_mix(frag16 &a, const float *X0, const float *X1, float c0, float c1) {
-    const int l = threadIdx.x & 63, r = l & 31, h = l >> 5;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int k = 2 * q + h, o = T ? k + LD * r : r + LD * k;
-        a[q] = c0 * X0[o] + c1 * X1[o];
-    }
-}
-template <bool T>
-__device__ __forceinline__ void bfrag_mix(frag16 &b, const float *X0, const float *X1, float c0, float c1) {
-    const int l = threadIdx.x & 63, j = l & 31, h = l >> 5;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int k = 2 * q + h, o = T ? j + LD * k : k + LD * j;
-        b[q] = c0 * X0[o] + c1 * X1[o];
     }
 }
 
@@ -410,14 +390,6 @@ __device__ void site0_32(const W32 &C, const float *c0, const float *Mb, double 
     *n2 = v[1];
 }
 
-// global identity (restricted to w) into a padded block
-__device__ __forceinline__ void g_identity(float *g, int w) {
-    for (int e = threadIdx.x; e < GB; e += MQE_NT) {
-        const int r = e & 31, c = e >> 5;
-        g[e] = (r == c && r < w) ? 1.f : 0.f;
-    }
-}
-
 // Energy sweep (right -> left) from the center phi_{L-1} (LDS pair pc); L_k in
 // global memory.  Returns <trial|H|phi>; *Ou = <trial|phi>; normalises site 0.
 __device__ double energy_sweep32(const W32 &C, int pc, double *On, double *Ou, bool canon) {
@@ -504,14 +476,13 @@ __device__ double energy_sweep32(const W32 &C, int pc, double *On, double *Ou, b
                     if ((lane & 15) == 0) Mq[(sp + 2 * (q >> 1)) * 4 + (ss + 2 * (q & 1))] = d;
                 }
                 float *R = C.blk(B_R), *Mk = C.blk(B_LM);
-                float *rg = C.Rg + (size_t)k * GB, *mg = C.Rg + (size_t)(L + 1 + k) * GB;
+                float *rg = C.Rg + (size_t)k * GB;
                 for (int e = threadIdx.x; e < GB; e += MQE_NT) {
                     const int o = (e & 31) + LD * (e >> 5);
-                    const float rv = V[o] + V[3 * BLK + o], mv = Mt[o] + Mt[BLK + o];
+                    const float rv = V[o] + V[3 * BLK + o];
                     R[o] = rv;
                     rg[e] = rv;
-                    Mk[o] = mv;
-                    mg[e] = mv;
+                    Mk[o] = Mt[o] + Mt[BLK + o];
                 }
                 if (k >= 2) {  // phi_{k-2} into the buffer of phi_k (not read in this stage), L_{k-2}
                     float *const d[3] = {cen, cen + BLK, C.blk(B_L)};
@@ -594,7 +565,6 @@ __device__ double energy_sweep32(const W32 &C, int pc, double *On, double *Ou, b
         }
         __syncthreads();
         l2g(C.blk(B_R), C.Rg + (size_t)k * GB);
-        g_identity(C.Rg + (size_t)(L + 1 + k) * GB, C.wv[k]);  // M_k = 1: sites >= k right-orthonormal
         double e = 0.0;
         for (int q = 0; q < 16; ++q) e += S.SS[q] * Mq[q];
         etot += S.Jc[k - 1] * e;
@@ -632,7 +602,7 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
     const int L = S.L;
     __shared__ int wv[MQE_MAXL + 1];
     __shared__ double red[2 * MQE_WAVES];
-    __shared__ double M[16], scal[4];
+    __shared__ double M[16];
     __shared__ double grid_s[4 * MAXG_T32];  // the Marsaglia grid
     __shared__ double tab_s[2][36];          // Trotter tables S, U, VT of the current and the next coupling
     for (int k = threadIdx.x; k <= L; k += MQE_NT) wv[k] = P.wvd[(size_t)slot * (L + 1) + k];
@@ -689,24 +659,21 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
     }
 
     // ---------------- propagation (MPSQMC2.cpp:532-600), h = 0 ----------------
-    // The tensor that becomes the next coupling's left site is normalised
-    // lazily: its squared norm is known one stage after the coupling ends and
-    // the factor sc = 1/norm is folded into the next coupling's mixed elements
-    // and sampled operator.
-    int pc = B_BUF0, pj = B_BUF1;
+    // The walker is not normalised between couplings: the right site's sampled
+    // operator is scaled by O / <trial|B(x) phi> = O / (4 f*), so the overlap
+    // stays O and the norm stays near 1 (every quantity of the step is a ratio
+    // in which the walker's scale cancels); the energy sweep takes the exact
+    // norm.  The center tensors rotate over three LDS block pairs: the current
+    // left site (pc), the right site (pj) and the next coupling's right site
+    // (pf), stored from registers fetched one coupling ahead.
+    int pc = B_BUF0, pj = B_BUF1, pf = B_BUF2;
     g2l(C.ph(0, 0), C.blk(pc));
     g2l(C.ph(0, 1), C.blk(pc) + BLK);
+    g2l(C.ph(1, 0), C.blk(pj));
+    g2l(C.ph(1, 1), C.blk(pj) + BLK);
+    g2l(C.Rg + (size_t)2 * GB, C.blk(B_R));
     set_unit(C.blk(B_L));
-    set_unit(C.blk(B_N));
-    double sc = 1.0, fstar = 0.0;
     const int Ng = S.Ng;
-    // operands of coupling c, fetched one coupling ahead: phi_{c+1}, R_{c+2}
-    // and the right norm environment M_{c+2}
-    float pre[16];
-    {
-        const float *const g[4] = {C.ph(1, 0), C.ph(1, 1), C.Rg + (size_t)2 * GB, C.Rg + (size_t)(L + 3) * GB};
-        pre_load<4>(pre, g);
-    }
     for (int e = threadIdx.x; e < 4 * Ng; e += MQE_NT) grid_s[e] = S.grid[e];
     // tables of coupling c: tab[0..3] = sqrt(S), tab[4..19] = U, tab[20..35] = VT
     auto table_entry = [&](int c, int t) {
@@ -714,6 +681,8 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
     };
     if (threadIdx.x < 36) tab_s[0][threadIdx.x] = table_entry(0, threadIdx.x);
     double tnext = (threadIdx.x < 36 && S.nC > 1) ? table_entry(1, threadIdx.x) : 0.0;
+    __syncthreads();
+    float pre[16];  // phi_{j+1} and R_{j+2}, the next coupling's walker operands
     for (int c = 0; c < S.nC; ++c) {
         const int i = c, j = c + 1;
         float *cen = C.blk(pc), *nxt = C.blk(pj);
@@ -728,22 +697,9 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
             load_frag(f0, C.tn(j, 0));
             load_frag(f1, C.tn(j, 1));
         }
-        {
-            float *const d[4] = {nxt, nxt + BLK, C.blk(B_R), C.blk(B_LM)};
-            pre_store<4>(pre, d);
-        }
-        if (c > 0 && threadIdx.x < 36) tab_s[c & 1][threadIdx.x] = tnext;
-        __syncthreads();
-        if (c + 1 < S.nC) {  // fetch the next coupling's operands
-            const float *const g[4] = {C.ph(j + 1, 0), C.ph(j + 1, 1), C.Rg + (size_t)(j + 2) * GB,
-                                       C.Rg + (size_t)(L + 3 + j) * GB};
-            pre_load<4>(pre, g);
-            if (c > 0 && threadIdx.x < 36) tnext = table_entry(c + 1, threadIdx.x);
-        }
-        if (c > 0) {  // squared norm of the previous coupling's result (stage G)
-            const double nrm = sqrt(scal[0] + scal[1] + scal[2] + scal[3]);
-            sc = 1.0 / nrm;
-            O = 4.0 * fstar / nrm;
+        if (c + 1 < S.nC) {
+            const float *const g[3] = {C.ph(j + 1, 0), C.ph(j + 1, 1), C.Rg + (size_t)(j + 2) * GB};
+            pre_load<3>(pre, g);
         }
         // stage AB, chained in registers:
         //   waves 0,1 (s = w):     Y[s] = L_i phi_i[s],        G(s',s) = T_i[s']^T Y[s]
@@ -813,7 +769,7 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
                         for (int t = 0; t < 2; ++t) mv += v[tp + 2 * t] * m[(xsp + 2 * tp) * 4 + (xss + 2 * t)];
                     sacc += u[xsp + 2 * xss] * mv;
                 }
-            cq = sS[a] * sS[bb] * (sacc * sc);
+            cq = sS[a] * sS[bb] * sacc;
         }
         double cb[16];
 #pragma unroll
@@ -920,7 +876,7 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
             if (!(pdfnorm > 0.0) || !isfinite(pdfnorm)) atomicOr(P.status, 1);
         }
         const double *xs = grid_s + 4 * gs;
-        fstar = 0.0;
+        double fstar = 0.0;  // <trial|B(x*) phi> / 4
 #pragma unroll
         for (int l = 0; l < 4; ++l) {
             double t = 0.0;
@@ -940,78 +896,49 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
                     Vx[up + 2 * dn] += pf * tVT[kk + 4 * (up + 2 * dn)];
                 }
         }
+        {  // right site scaled so that the overlap stays O
+            const double alpha = O / (4.0 * fstar);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) Ux[q] *= sc;  // left site carries the deferred normalisation
-        // stage F: the sampled operators applied on the fly (phi' = Ux phi_i on the
-        // left site, Vx phi_j on the right one), chained in registers:
-        //   waves 0,1 (s):     X[s] = N_i phi'_i[s],       N^(s) = phi'_i[s]^T X[s];  phi'_i[s] to global
-        //   waves 2,3 (t):     Z[t] = M_{j+1} phi'_j[t]^T, K^(t)T = phi'_j[t] Z[t]
-        //   all:               L_{i+1} = sum_{s',s} Ux(s',s) G(s',s) (LDS and global)
+            for (int q = 0; q < 4; ++q) Vx[q] *= alpha;
+        }
+        // stage F (elementwise): L_{i+1} = sum_{s',s} Ux(s',s) G(s',s) (LDS and global),
+        // phi'_i = Ux phi_i to global, phi'_j = Vx phi_j in place (the next
+        // coupling's left site); the next coupling's right site and R_{j+2}
+        // from the registers into the free pair and B_R
         {
             const float *G = C.blk(B_Y);
             float *Ln = C.blk(B_L), *lg = C.Lg + (size_t)(i + 1) * GB;
+            float *pg0 = C.ph(i, 0), *pg1 = C.ph(i, 1);
             const float u0 = (float)Ux[0], u1 = (float)Ux[1], u2 = (float)Ux[2], u3 = (float)Ux[3];
             for (int e = threadIdx.x; e < GB; e += MQE_NT) {
                 const int o = (e & 31) + LD * (e >> 5);
                 const float v = u0 * G[o] + u2 * G[BLK + o] + u1 * G[2 * BLK + o] + u3 * G[3 * BLK + o];
                 Ln[o] = v;
                 lg[e] = v;
+                const float a0 = cen[o], a1 = cen[BLK + o];
+                pg0[e] = u0 * a0 + u2 * a1;
+                pg1[e] = u1 * a0 + u3 * a1;
             }
-            v16f a = zero16();
-            if (w < 2) {
-                const float c0 = (float)Ux[w], c1 = (float)Ux[w + 2];
-                frag16 pb, xb, pa;
-                bfrag_mix<false>(pb, cen, cen + BLK, c0, c1);
-                v16f x = zero16();
-                mma_lr<false>(x, C.blk(B_N), LD, pb);
-                acc_to_bfrag(x, xb);
-                afrag_mix<true>(pa, cen, cen + BLK, c0, c1);
-                mma_rr(a, pa, xb);
-                float *pg = C.ph(i, w);
-#pragma unroll
-                for (int n = 0; n < 16; ++n) {
-                    const int e = lane + 64 * n, o = (e & 31) + LD * (e >> 5);
-                    pg[e] = c0 * cen[o] + c1 * cen[BLK + o];
+            apply2(nxt, Vx);
+            if (c + 1 < S.nC) {
+                float *const d[3] = {C.blk(pf), C.blk(pf) + BLK, C.blk(B_R)};
+                pre_store<3>(pre, d);
+                if (threadIdx.x < 36) {
+                    tab_s[(c + 1) & 1][threadIdx.x] = tnext;
+                    if (c + 2 < S.nC) tnext = table_entry(c + 2, threadIdx.x);
                 }
-            } else {
-                const int t = w - 2;
-                const float c0 = (float)Vx[t], c1 = (float)Vx[t + 2];
-                frag16 pb, zb, pa;
-                bfrag_mix<true>(pb, nxt, nxt + BLK, c0, c1);
-                v16f z = zero16();
-                mma_lr<false>(z, C.blk(B_LM), LD, pb);
-                acc_to_bfrag(z, zb);
-                afrag_mix<false>(pa, nxt, nxt + BLK, c0, c1);
-                mma_rr(a, pa, zb);
             }
-            store(a, C.blk(B_V) + w * BLK, LD);
         }
         __syncthreads();
-        // stage G: N_{i+1} = N^(0) + N^(1); squared norm <N_{i+1}, K^(0)T + K^(1)T> = tr(N_{i+1} K),
-        // one partial per wave (read after the next barrier); phi'_j = Vx phi_j in place
-        {
-            const float *V = C.blk(B_V);
-            float *Nn = C.blk(B_N);
-            float part = 0.f;
-            for (int e = threadIdx.x; e < GB; e += MQE_NT) {
-                const int o = (e & 31) + LD * (e >> 5);
-                const float n = V[o] + V[BLK + o];
-                Nn[o] = n;
-                part += n * (V[2 * BLK + o] + V[3 * BLK + o]);
-            }
-            const double ws = wave_sum((double)part);
-            if (lane == 0) scal[w] = ws;
-            apply2(nxt, Vx);
-        }
-        int t = pc; pc = pj; pj = t;
+        const int t = pc; pc = pj; pj = pf; pf = t;
     }
-    __syncthreads();
-    {
-        const double nrm = sqrt(scal[0] + scal[1] + scal[2] + scal[3]);
-        O = 4.0 * fstar / nrm;
-        scale_blocks(C.blk(pc), 2, 1.0 / nrm);
+    if (pc == B_BUF2) {  // the energy sweep takes the center in B_BUF0 / B_BUF1
+        const float *src = C.blk(B_BUF2);
+        float *dst = C.blk(B_BUF0);
+        for (int e = threadIdx.x; e < 2 * BLK; e += MQE_NT) dst[e] = src[e];
+        pc = B_BUF0;
+        __syncthreads();
     }
-    __syncthreads();
     // ---------------- local energy (Walker.cpp:72-77) ----------------
     double On, Ou;
     const bool canon = canon_walker(ctl, slot, P.since[slot]);
