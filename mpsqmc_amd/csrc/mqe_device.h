@@ -471,6 +471,176 @@ __device__ __forceinline__ void gemm(int m, int n, int k, const T *A, int lda, c
 }
 
 // ---------------------------------------------------------------------------
+// Two chained FP32 matrix-core products with the intermediate in registers:
+//   F_f = A op(B_f)            (m1 x n1, inner dimension k1; f < NF, NF = 1, 2)
+//   D   = sum_f op(C_f) F_f    (mD x n1, inner dimension m1 <= 64, mD <= 64)
+// for nu independent units (operands of unit u from get(u)), whole workgroup,
+// no barrier inside.  A wave takes (unit, 32-column strip) items: for each f
+// it forms the strip of F_f in two 32 x 32 accumulator tiles (rows >= m1
+// zeroed) and feeds the accumulator slots straight into the second chain as
+// its B operand, accumulating the (at most two) row tiles of the D strip in
+// registers.  The second chain takes its inner index in the order the
+// accumulators hold it: MFMA q of F row tile t uses rows 32t + 8(q/4) + 4h +
+// q%4 (h = lane half), exactly accumulator slot q of lane (column, h), so no
+// lane exchange is needed; op(C) is loaded in the same order.
+// Items go to waves round-robin starting at wave0; the return value is the
+// wave that takes the next item, so several independent calls can share the
+// waves between two barriers.
+// TB: B_f stored n1 x k1 (op(B) = B^T).  TC: C_f stored m1 x mD (op(C) = C^T,
+// contiguous along the inner index), else mD x m1.  All matrices column-major.
+// ---------------------------------------------------------------------------
+struct ChainOps {
+    const float *A;      // m1 x k1, leading dimension lda
+    const float *B[2];   // B_f
+    const float *C[2];   // C_f
+    float *D;            // mD x n1, leading dimension ldd
+};
+
+typedef float v16f_c __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ v16f_c mfma32(float a, float b, v16f_c c) {
+    return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+}
+
+// Buffer resources over one walker's or the trial's arrays: the 64-bit base in
+// scalar registers and 32-bit byte offsets per lane.  An offset at or beyond
+// BUF_RANGE reads 0 and drops a store (the hardware range check).
+constexpr int BUF_RANGE = 0x7ffffff0;
+constexpr int BUF_OOB = 0x7ffffff8;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *p) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, BUF_RANGE, 0x00020000);
+}
+// (the b32 / b128 builtins move raw 32-bit words: bit casts, not conversions)
+__device__ __forceinline__ float buf_ld(__amdgpu_buffer_rsrc_t r, int voff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, 0, 0));
+}
+__device__ __forceinline__ void buf_st(float v, __amdgpu_buffer_rsrc_t r, int voff, int imm) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, voff, imm, 0);
+}
+__device__ __forceinline__ void buf_ld4(__amdgpu_buffer_rsrc_t r, int voff, float *v) {
+    const float4 x = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, 0, 0));
+    v[0] = x.x;
+    v[1] = x.y;
+    v[2] = x.z;
+    v[3] = x.w;
+}
+
+template <bool TB, bool TC, int NF, class Get>
+__device__ __forceinline__ int chain2(int wave0, int nu, int m1, int n1, int k1, int mD, int lda, int ldb, int ldc,
+                                      int ldd, Get get) {
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nstrip = (n1 + 31) >> 5, nitems = nu * nstrip;
+    const bool two = m1 > 32;     // second row tile of F
+    const bool twoD = mD > 32;    // second row tile of D
+    const int sa = 4 * lda, sb = TB ? 4 * ldb : 4;  // bytes per k step of A and op(B)
+    const int kf = k1 & ~15;      // whole 16-deep slices
+#pragma nounroll
+    for (int it = (w - wave0 + 2 * MQE_WAVES) % MQE_WAVES; it < nitems; it += MQE_WAVES) {
+        const int u = it / nstrip, c0 = (it - u * nstrip) * 32;
+        const ChainOps op = get(u);
+        // the lane coordinates as opaque values inside the loop: the per-element
+        // offsets below are then formed next to their loads instead of being
+        // hoisted out of the loop and held in registers throughout
+        int r = lane & 31, h = lane >> 5;
+        asm volatile("" : "+v"(r), "+v"(h));
+        v16f_c D0, D1;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) D0[e] = D1[e] = 0.f;
+#pragma nounroll
+        for (int f = 0; f < NF; ++f) {
+            asm volatile("" : "+v"(r), "+v"(h));  // (the same for the loop over f)
+            const int col = min(c0 + r, n1 - 1);
+            const int vA0 = 4 * min(r, m1 - 1), vA1 = 4 * min(32 + r, m1 - 1);
+            const int vB = TB ? 4 * col : 4 * ldb * col;
+            const int row0 = min(r, mD - 1), row1 = min(32 + r, mD - 1);
+            // ---- first chain: the F_f strip; lane (r, h) takes k = k0 + 8h + q of the
+            // slice at k0; in the last, partial slice k >= k1 reads 0 ----
+            v16f_c F0, F1;
+#pragma unroll
+            for (int e = 0; e < 16; ++e) F0[e] = F1[e] = 0.f;
+            const auto rA = buf_rsrc(op.A), rB = buf_rsrc(op.B[f]);
+            auto slice = [&](int k0, bool tail) {
+                float a0[8], a1[8], bv[8];
+                const int kl = k0 + 8 * h;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const bool ok = !tail || kl + q < k1;
+                    a0[q] = buf_ld(rA, ok ? vA0 + sa * (kl + q) : BUF_OOB);
+                    bv[q] = buf_ld(rB, ok ? vB + sb * (kl + q) : BUF_OOB);
+                }
+                if (two) {
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) a1[q] = buf_ld(rA, !tail || kl + q < k1 ? vA1 + sa * (kl + q) : BUF_OOB);
+                }
+#pragma unroll
+                for (int q = 0; q < 8; ++q) F0 = mfma32(a0[q], bv[q], F0);
+                if (two) {
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) F1 = mfma32(a1[q], bv[q], F1);
+                }
+            };
+#pragma nounroll
+            for (int k0 = 0; k0 < kf; k0 += 16) slice(k0, false);
+            if (kf < k1) slice(kf, true);
+            if (m1 & 31) {  // rows of F beyond m1 (clamped A rows) must not contribute
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    const int row = 8 * (e >> 2) + 4 * h + (e & 3);
+                    if (row >= m1) F0[e] = 0.f;
+                    if (32 + row >= m1) F1[e] = 0.f;
+                }
+            }
+            // ---- second chain: D (+)= op(C_f) F_f, row tile by row tile ----
+            const auto rC = buf_rsrc(op.C[f]);
+            const bool vecC = TC && (m1 & 3) == 0 && (ldc & 3) == 0 && aligned16(op.C[f], op.C[f], 0, 0);
+            auto chunk = [&](int row, int t, float (&cv)[16]) {
+                if (TC && vecC) {
+                    // a run at or beyond m1 meets zero rows of F: any values of C do
+#pragma unroll
+                    for (int g = 0; g < 4; ++g)
+                        buf_ld4(rC, 4 * (ldc * row + min(32 * t + 8 * g + 4 * h, m1 - 4)), cv + 4 * g);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 16; ++e) {
+                        const int k = min(32 * t + 8 * (e >> 2) + 4 * h + (e & 3), m1 - 1);
+                        cv[e] = buf_ld(rC, TC ? 4 * (ldc * row + k) : 4 * (row + ldc * k));
+                    }
+                }
+            };
+            auto rowtile = [&](int row, v16f_c &Dq) {
+                float c0v[16], c1v[16];
+                chunk(row, 0, c0v);
+                if (two) chunk(row, 1, c1v);
+#pragma unroll
+                for (int q = 0; q < 16; ++q) Dq = mfma32(c0v[q], F0[q], Dq);
+                if (two) {
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) Dq = mfma32(c1v[q], F1[q], Dq);
+                }
+            };
+            rowtile(row0, D0);
+            if (twoD) rowtile(row1, D1);
+        }
+        // rows 32 qt + 8(e/4) + 4h + e%4 of column c0 + r; outside D the store is dropped
+        const auto rD = buf_rsrc(op.D);
+        const int vd = c0 + r < n1 ? 4 * (ldd * (c0 + r) + 4 * h) : BUF_OOB;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const int rr = 8 * (e >> 2) + 4 * h + (e & 3), imm = 4 * (8 * (e >> 2) + (e & 3));
+            buf_st(D0[e], rD, rr < mD ? vd : BUF_OOB, imm);
+        }
+        if (twoD) {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const int rr = 32 + 8 * (e >> 2) + 4 * h + (e & 3), imm = 4 * (32 + 8 * (e >> 2) + (e & 3));
+                buf_st(D1[e], rD, rr < mD ? vd : BUF_OOB, imm);
+            }
+        }
+    }
+    return (wave0 + nitems) % MQE_WAVES;
+}
+
+// ---------------------------------------------------------------------------
 // Householder QR (LAPACK geqrf/orgqr conventions), FP64, whole workgroup.
 // A: m x n (m >= n), column c at Ac(c) (columns c < c_split at A + m c, the
 // others at B + m (c - c_split): the caller may keep the trailing columns,

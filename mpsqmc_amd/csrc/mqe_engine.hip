@@ -94,6 +94,7 @@ struct Sys {
     double h;
     double ssp[2];                   // exp(h dtau m / 2)
     int tiles[8];                    // FP32 product tile shape by walker bond bucket (max bond 1-16, 17-32, ...)
+    const float *trialT;             // FP32 general path: T_k[s]^T (t_{k+1} x t_k) at toff[k] + s t_k t_{k+1}
 };
 
 // bond bucket of a walker: its largest bond in steps of 16 (0: <= 16, ..., 7: > 112)
@@ -175,7 +176,9 @@ __host__ __device__ inline long scratch_bytes_for(int Dt, int Dw, int Ng) {
 // ---------------------------------------------------------------------------
 // per-walker context
 // ---------------------------------------------------------------------------
-template <typename T>
+// CH: the FP32 products are chained in registers (chain2); the host selects
+// this instance only when every trial and walker bond is at most 64.
+template <typename T, bool CH>
 struct Ctx {
     const Sys *S;
     const T *trial;
@@ -190,32 +193,95 @@ struct Ctx {
     long qlds_doubles;   // its size
     int tile;            // output tile shape of the FP32 matrix-core products (TILE_*)
 
+    const float *trialT; // FP32: the trial with every T_k[s] transposed (t_{k+1} x t_k), or nullptr
+
     __device__ T *site(int k) const { return phi + woff[k]; }
     __device__ const T *tsite(int k) const { return trial + S->toff[k]; }
+    __device__ const float *tsiteT(int k) const { return trialT + S->toff[k]; }
+
     __device__ T *Lk(int k) const { return Lenv + eoff[k]; }
     __device__ T *Rk(int k) const { return Renv + eoff[k]; }
     __device__ T *Mk(int k) const { return Renv + moff[k]; }  // right norm environment (w_k x w_k)
 };
 
+// G(s', s) = T_i[s']^T L_i phi_i[s] (t_{i+1} x w_{i+1}) chained in registers into
+// out + s' ssp + s ss (ld ldo); false (nothing done) where the chained
+// products do not apply.
+template <typename T, bool CH>
+__device__ __forceinline__ bool compute_G_chained(const Ctx<T, CH> &C, int i, const T *Lenv_i, T *out, int ldo, long ssp,
+                                                  long ss) {
+    if constexpr (sizeof(T) == sizeof(float)) {
+        const int ti = C.tv[i], tn = C.tv[i + 1], wi = C.wv[i], wn = C.wv[i + 1];
+        if constexpr (CH) {
+            const float *Ti = (const float *)C.tsite(i), *ph = (const float *)C.site(i);
+            chain2<false, true, 1>(0, 4, ti, wn, wi, tn, ti, wi, ti, ldo, [&](int u) {
+                const int sp = u >> 1, s = u & 1;
+                ChainOps o;
+                o.A = (const float *)Lenv_i;
+                o.B[0] = o.B[1] = ph + (size_t)s * wi * wn;
+                o.C[0] = o.C[1] = Ti + (size_t)sp * ti * tn;
+                o.D = (float *)out + (size_t)sp * ssp + (size_t)s * ss;
+                return o;
+            });
+            __syncthreads();
+            return true;
+        }
+    }
+    return false;
+}
+
 // Y = L_i phi_i (t_i x 2 w_{i+1});  G = T_i^T Y (2 t_{i+1} x 2 w_{i+1}), block
 // (s', s) = T_i[s']^T L_i phi_i[s] at G + s' t + s w (2t), ld 2t.
-template <typename T>
-__device__ __forceinline__ void compute_G(const Ctx<T> &C, int i, const T *Lenv_i) {
+template <typename T, bool CH>
+__device__ __forceinline__ void compute_G(const Ctx<T, CH> &C, int i, const T *Lenv_i) {
     const int ti = C.tv[i], tn = C.tv[i + 1], wi = C.wv[i], wn = C.wv[i + 1];
     T *Y = C.ts + C.sl.Y, *G = C.ts + C.sl.G;
+    if (compute_G_chained(C, i, Lenv_i, G, 2 * tn, tn, (long)wn * 2 * tn)) return;
     gemm<false, false, T>(ti, 2 * wn, wi, Lenv_i, ti, C.site(i), wi, Y, ti, false, C.tile);
     __syncthreads();
     gemm<true, false, T>(2 * tn, 2 * wn, ti, C.tsite(i), ti, Y, ti, G, 2 * tn, false, C.tile);
     __syncthreads();
 }
 
+// The four blocks G(s', s) of bond i+1 as contiguous channels ch = 2 s' + s
+// (t_{i+1} x w_{i+1}, ld t_{i+1}) at A.
+template <typename T, bool CH>
+__device__ __forceinline__ void compute_G_channels(const Ctx<T, CH> &C, int i, T *A) {
+    const int tn = C.tv[i + 1], wn = C.wv[i + 1];
+    if (compute_G_chained(C, i, C.Lk(i), A, tn, 2L * tn * wn, (long)tn * wn)) return;
+    compute_G(C, i, C.Lk(i));
+    const T *G = C.ts + C.sl.G;
+    for (int e = threadIdx.x; e < 4 * tn * wn; e += MQE_NT) {
+        const int ch = e / (tn * wn), r = e % (tn * wn), a = r % tn, b = r / tn;
+        const int sp = ch >> 1, s = ch & 1;
+        A[e] = G[(size_t)sp * tn + (size_t)s * wn * 2 * tn + a + (size_t)2 * tn * b];
+    }
+    __syncthreads();
+}
+
 // W[t] = R_{j+1} phi_j[t]^T (t_{j+1} x w_j);  V[t'] = T_j[t'] W (t_j x 2 w_j);
 // block (t', t) at V + t' (t_j 2 w_j) + t (t_j w_j), ld t_j.
-template <typename T>
-__device__ __forceinline__ void compute_V(const Ctx<T> &C, int j) {
+template <typename T, bool CH>
+__device__ __forceinline__ void compute_V(const Ctx<T, CH> &C, int j) {
     const int tj = C.tv[j], tn = C.tv[j + 1], wj = C.wv[j], wn = C.wv[j + 1];
     T *W = C.ts + C.sl.W, *V = C.ts + C.sl.V;
     const T *R = C.Rk(j + 1), *ph = C.site(j);
+    if constexpr (sizeof(T) == sizeof(float)) {
+        if constexpr (CH) {
+            const float *TTj = C.tsiteT(j);
+            chain2<true, true, 1>(0, 4, tn, wj, wn, tj, tn, wj, tn, tj, [&](int u) {
+                const int tp = u >> 1, t = u & 1;
+                ChainOps o;
+                o.A = (const float *)R;
+                o.B[0] = o.B[1] = (const float *)ph + (size_t)t * wj * wn;
+                o.C[0] = o.C[1] = TTj + (size_t)tp * tj * tn;
+                o.D = (float *)V + (size_t)tp * tj * 2 * wj + (size_t)t * tj * wj;
+                return o;
+            });
+            __syncthreads();
+            return;
+        }
+    }
     gemm_batch<false, true, T>(2, tn, wj, wn, R, tn, 0, ph, wj, (long)wj * wn, W, tn, (long)tn * wj, false, C.tile);
     __syncthreads();
     const T *Tj = C.tsite(j);
@@ -225,12 +291,29 @@ __device__ __forceinline__ void compute_V(const Ctx<T> &C, int j) {
 }
 
 // Carry four channels (t_k x w_k, ld t_k, contiguous at src) through site k
-// into dst (t_{k+1} x w_{k+1}).  Uses Y as temporary.
-template <typename T>
-__device__ __forceinline__ void carry_channels(const Ctx<T> &C, int k, const T *src, T *dst) {
+// into dst (t_{k+1} x w_{k+1}): dst_ch = sum_s T_k[s]^T src_ch phi_k[s].
+template <typename T, bool CH>
+__device__ __forceinline__ void carry_channels(const Ctx<T, CH> &C, int k, const T *src, T *dst) {
     const int tk = C.tv[k], tn = C.tv[k + 1], wk = C.wv[k], wn = C.wv[k + 1];
-    T *Y = C.ts + C.sl.Y4;  // Y_ch = src_ch phi_k (tk x 2 wn), the four channels one after the other
     const T *Tk = C.tsite(k);
+    if constexpr (sizeof(T) == sizeof(float)) {
+        if constexpr (CH) {
+            const float *ph = (const float *)C.site(k);
+            chain2<false, true, 2>(0, 4, tk, wn, wk, tn, tk, wk, tk, tn, [&](int ch) {
+                ChainOps o;
+                o.A = (const float *)src + (size_t)ch * tk * wk;
+                o.B[0] = ph;
+                o.B[1] = ph + (size_t)wk * wn;
+                o.C[0] = (const float *)Tk;
+                o.C[1] = (const float *)Tk + (size_t)tk * tn;
+                o.D = (float *)dst + (size_t)ch * tn * wn;
+                return o;
+            });
+            __syncthreads();
+            return;
+        }
+    }
+    T *Y = C.ts + C.sl.Y4;  // Y_ch = src_ch phi_k (tk x 2 wn), the four channels one after the other
     const long sy = (long)tk * 2 * wn;
     gemm_batch<false, false, T>(4, tk, 2 * wn, wk, src, tk, (long)tk * wk, C.site(k), wk, 0, Y, tk, sy, false, C.tile);
     __syncthreads();
@@ -242,8 +325,8 @@ __device__ __forceinline__ void carry_channels(const Ctx<T> &C, int k, const T *
 
 // Two-site mixed tensor M[(s'+2t')*4 + (s+2t)] = <G(s',s), V(t',t)> at a bond
 // with trial dim tb and walker dim wb.  g[ch], ch = 2 s' + s, with leading dim gld.
-template <typename T>
-__device__ __forceinline__ void pair_tensor(const Ctx<T> &C, int tb, int wb, const T *const g[4], int gld, double *M) {
+template <typename T, bool CH>
+__device__ __forceinline__ void pair_tensor(const Ctx<T, CH> &C, int tb, int wb, const T *const g[4], int gld, double *M) {
     const T *V = C.ts + C.sl.V;
     double acc[16];
 #pragma unroll
@@ -277,28 +360,23 @@ __device__ __forceinline__ void pair_tensor(const Ctx<T> &C, int tb, int wb, con
 }
 
 // Mixed tensor for the pair (i, j) from L_i, sites i..j and R_{j+1}.
-template <typename T>
-__device__ __forceinline__ void pair_for_coupling(const Ctx<T> &C, int i, int j, double *M) {
-    compute_G(C, i, C.Lk(i));
+template <typename T, bool CH>
+__device__ __forceinline__ void pair_for_coupling(const Ctx<T, CH> &C, int i, int j, double *M) {
     const int tn = C.tv[i + 1], wn = C.wv[i + 1];
-    const T *G = C.ts + C.sl.G;
     const T *g[4];
     int gld;
     if (j == i + 1) {
+        compute_G(C, i, C.Lk(i));
+        const T *G = C.ts + C.sl.G;
         for (int ch = 0; ch < 4; ++ch) {
             const int sp = ch >> 1, s = ch & 1;
             g[ch] = G + (size_t)sp * tn + (size_t)s * wn * 2 * tn;
         }
         gld = 2 * tn;
     } else {
-        // copy the four blocks into contiguous channels, then carry to bond j
+        // the four blocks as contiguous channels (ch = 2 s' + s), carried to bond j
         T *A = C.ts + C.sl.GcA, *B = C.ts + C.sl.GcB;
-        for (int e = threadIdx.x; e < 4 * tn * wn; e += MQE_NT) {
-            const int ch = e / (tn * wn), r = e % (tn * wn), a = r % tn, b = r / tn;
-            const int sp = ch >> 1, s = ch & 1;
-            A[e] = G[(size_t)sp * tn + (size_t)s * wn * 2 * tn + a + (size_t)2 * tn * b];
-        }
-        __syncthreads();
+        compute_G_channels(C, i, A);
         for (int k = i + 1; k < j; ++k) {
             carry_channels(C, k, A, B);
             T *t = A; A = B; B = t;
@@ -312,9 +390,26 @@ __device__ __forceinline__ void pair_for_coupling(const Ctx<T> &C, int i, int j,
 }
 
 // L_{k+1} = sum_s T_k[s]^T L_k phi_k[s]
-template <typename T>
-__device__ __forceinline__ void advance_L(const Ctx<T> &C, int k) {
+template <typename T, bool CH>
+__device__ __forceinline__ void advance_L(const Ctx<T, CH> &C, int k) {
     const int tk = C.tv[k], tn = C.tv[k + 1], wk = C.wv[k], wn = C.wv[k + 1];
+    if constexpr (sizeof(T) == sizeof(float)) {
+        if constexpr (CH) {
+            const float *ph = (const float *)C.site(k), *Tk = (const float *)C.tsite(k);
+            chain2<false, true, 2>(0, 1, tk, wn, wk, tn, tk, wk, tk, tn, [&](int) {
+                ChainOps o;
+                o.A = (const float *)C.Lk(k);
+                o.B[0] = ph;
+                o.B[1] = ph + (size_t)wk * wn;
+                o.C[0] = Tk;
+                o.C[1] = Tk + (size_t)tk * tn;
+                o.D = (float *)C.Lk(k + 1);
+                return o;
+            });
+            __syncthreads();
+            return;
+        }
+    }
     T *Y = C.ts + C.sl.Y;
     gemm<false, false, T>(tk, 2 * wn, wk, C.Lk(k), tk, C.site(k), wk, Y, tk, false, C.tile);
     __syncthreads();
@@ -325,8 +420,26 @@ __device__ __forceinline__ void advance_L(const Ctx<T> &C, int k) {
 }
 
 // R_k = sum_t T_k[t] R_{k+1} phi_k[t]^T (the diagonal blocks of V at bond k)
-template <typename T>
-__device__ __forceinline__ void rebuild_R(const Ctx<T> &C, int k) {
+template <typename T, bool CH>
+__device__ __forceinline__ void rebuild_R(const Ctx<T, CH> &C, int k) {
+    if constexpr (sizeof(T) == sizeof(float)) {
+        const int tk = C.tv[k], tn = C.tv[k + 1], wk = C.wv[k], wn = C.wv[k + 1];
+        if constexpr (CH) {
+            const float *ph = (const float *)C.site(k), *TTk = C.tsiteT(k);
+            chain2<true, true, 2>(0, 1, tn, wk, wn, tk, tn, wk, tn, tk, [&](int) {
+                ChainOps o;
+                o.A = (const float *)C.Rk(k + 1);
+                o.B[0] = ph;
+                o.B[1] = ph + (size_t)wk * wn;
+                o.C[0] = TTk;
+                o.C[1] = TTk + (size_t)tk * tn;
+                o.D = (float *)C.Rk(k);
+                return o;
+            });
+            __syncthreads();
+            return;
+        }
+    }
     compute_V(C, k);
     const int tk = C.tv[k], wk = C.wv[k];
     const T *V = C.ts + C.sl.V;
@@ -337,12 +450,27 @@ __device__ __forceinline__ void rebuild_R(const Ctx<T> &C, int k) {
 
 // Right norm environment M_k = sum_s phi_k[s] M_{k+1} phi_k[s]^T (w_k x w_k):
 // the norm of the walker's part right of bond k in the current gauge.
-template <typename T>
-__device__ __forceinline__ void advance_M(const Ctx<T> &C, int k) {
+template <typename T, bool CH>
+__device__ __forceinline__ void advance_M(const Ctx<T, CH> &C, int k) {
     const int wk = C.wv[k], wn = C.wv[k + 1];
     T *X = C.ts + C.sl.Y;
     const T *ph = C.site(k), *Mn = C.Mk(k + 1);
     T *Mk = C.Mk(k);
+    if constexpr (sizeof(T) == sizeof(float)) {
+        if constexpr (CH) {
+            // M_{k+1} is symmetric: M_k = sum_s phi_k[s] (M_{k+1} phi_k[s]^T)
+            chain2<true, false, 2>(0, 1, wn, wk, wn, wk, wn, wk, wk, wk, [&](int) {
+                ChainOps o;
+                o.A = (const float *)Mn;
+                o.B[0] = o.C[0] = (const float *)ph;
+                o.B[1] = o.C[1] = (const float *)ph + (size_t)wk * wn;
+                o.D = (float *)Mk;
+                return o;
+            });
+            __syncthreads();
+            return;
+        }
+    }
     gemm_batch<false, false, T>(2, wk, wn, wn, ph, wk, (long)wk * wn, Mn, wn, 0, X, wk, (long)wk * wn, false, C.tile);
     __syncthreads();
     gemm2<false, true, T>(wk, wk, wn, X, X + (size_t)wk * wn, wk, ph, ph + (size_t)wk * wn, wk, Mk, wk, C.tile);
@@ -351,8 +479,8 @@ __device__ __forceinline__ void advance_M(const Ctx<T> &C, int k) {
 
 // Squared norm with the left norm environment N (bond k) and the right norm
 // environment M_{k+1}: sum_s tr(phi_k[s]^T N phi_k[s] M_{k+1}).
-template <typename T>
-__device__ __forceinline__ double norm2_at_M(const Ctx<T> &C, int k, const T *N) {
+template <typename T, bool CH>
+__device__ __forceinline__ double norm2_at_M(const Ctx<T, CH> &C, int k, const T *N) {
     const int wk = C.wv[k], wn = C.wv[k + 1];
     T *Z = C.ts + C.sl.Y, *Z2 = C.ts + C.sl.TMP;
     const T *ph = C.site(k), *Mn = C.Mk(k + 1);
@@ -365,8 +493,8 @@ __device__ __forceinline__ double norm2_at_M(const Ctx<T> &C, int k, const T *N)
     return block_sum(acc, C.red);
 }
 
-template <typename T>
-__device__ __forceinline__ void scale_site(const Ctx<T> &C, int k, double f) {
+template <typename T, bool CH>
+__device__ __forceinline__ void scale_site(const Ctx<T, CH> &C, int k, double f) {
     T *ph = C.site(k);
     const int n = C.wv[k] * C.wv[k + 1] * 2;
     for (int e = threadIdx.x; e < n; e += MQE_NT) ph[e] = (T)((double)ph[e] * f);
@@ -374,8 +502,8 @@ __device__ __forceinline__ void scale_site(const Ctx<T> &C, int k, double f) {
 }
 
 // phi_k[s] <- sum_s0 O[s + 2 s0] phi_k[s0]
-template <typename T>
-__device__ __forceinline__ void apply_op(const Ctx<T> &C, int k, const double *O) {
+template <typename T, bool CH>
+__device__ __forceinline__ void apply_op(const Ctx<T, CH> &C, int k, const double *O) {
     T *ph = C.site(k);
     const int blk = C.wv[k] * C.wv[k + 1];
     const T o00 = (T)O[0], o10 = (T)O[1], o01 = (T)O[2], o11 = (T)O[3];
@@ -390,8 +518,8 @@ __device__ __forceinline__ void apply_op(const Ctx<T> &C, int k, const double *O
 // LQ of site k (w_k x 2 w_{k+1}) in FP64: phi_k <- Q (right-orthonormal),
 // phi_{k-1} <- phi_{k-1} Lmat, then R_k = sum_t T_k[t] R_{k+1} Q[t]^T.
 // Leaves V (all four blocks) computed for bond k.
-template <typename T>
-__device__ __forceinline__ void lq_site(const Ctx<T> &C, int k) {
+template <typename T, bool CH>
+__device__ __forceinline__ void lq_site(const Ctx<T, CH> &C, int k) {
     const int wk = C.wv[k], wn = C.wv[k + 1], wp = C.wv[k - 1];
     const int m = 2 * wn, n = wk;
     // the FP64 factorisation's sequential column steps are latency-bound: the
@@ -435,8 +563,8 @@ __device__ __forceinline__ void lq_site(const Ctx<T> &C, int k) {
 }
 
 // <trial|phi> and sum_s sz[s] <trial|S^z_0|phi> at site 0 (L_0 = 1, right part via R_1).
-template <typename T>
-__device__ __forceinline__ void site0_overlap(const Ctx<T> &C, double *ov, double *szv) {
+template <typename T, bool CH>
+__device__ __forceinline__ void site0_overlap(const Ctx<T, CH> &C, double *ov, double *szv) {
     const int t1 = C.tv[1], w1 = C.wv[1];
     const T *T0 = C.tsite(0), *p0 = C.site(0), *R1 = C.Rk(1);
     double a[2] = {0.0, 0.0};
@@ -458,8 +586,8 @@ __device__ __forceinline__ void set_one(T *p) {
 
 // Right-canonicalise the whole walker with R environments, normalise at site 0.
 // Returns the (sign-fixed) overlap of the normalised state.
-template <typename T>
-__device__ __forceinline__ double right_canonicalize(const Ctx<T> &C) {
+template <typename T, bool CH>
+__device__ __forceinline__ double right_canonicalize(const Ctx<T, CH> &C) {
     const int L = C.S->L;
     set_one(C.Rk(L));
     for (int k = L - 1; k >= 1; --k) lq_site(C, k);
@@ -483,8 +611,8 @@ __device__ __forceinline__ double right_canonicalize(const Ctx<T> &C) {
 // Energy sweep from the right end (L_k valid for all k), see file header.
 // Returns e_tot = <trial|H|phi> of the state as it was on entry and writes the
 // overlap of the normalised state to *O_out, <trial|phi> before normalisation to *Ou.
-template <typename T>
-__device__ __forceinline__ double energy_sweep(const Ctx<T> &C, double *O_out, double *Ou, bool canon) {
+template <typename T, bool CH>
+__device__ __forceinline__ double energy_sweep(const Ctx<T, CH> &C, double *O_out, double *Ou, bool canon) {
     const Sys &S = *C.S;
     const int L = S.L;
     __shared__ double M[16];
@@ -523,12 +651,12 @@ __device__ __forceinline__ double energy_sweep(const Ctx<T> &C, double *O_out, d
         }
         for (int q = S.cs_ptr[k]; q < S.cs_ptr[k + 1]; ++q) {
             const int c = S.cs_idx[q], i = S.first[c];
-            compute_G(C, i, C.Lk(i));
             const int tn = C.tv[i + 1], wn = C.wv[i + 1];
-            const T *G = C.ts + C.sl.G;
             const T *g[4];
             int gld;
             if (i == k - 1) {
+                compute_G(C, i, C.Lk(i));
+                const T *G = C.ts + C.sl.G;
                 for (int ch = 0; ch < 4; ++ch) {
                     const int sp = ch >> 1, s = ch & 1;
                     g[ch] = G + (size_t)sp * tn + (size_t)s * wn * 2 * tn;
@@ -536,12 +664,7 @@ __device__ __forceinline__ double energy_sweep(const Ctx<T> &C, double *O_out, d
                 gld = 2 * tn;
             } else {
                 T *A = C.ts + C.sl.GcA, *B = C.ts + C.sl.GcB;
-                for (int e = threadIdx.x; e < 4 * tn * wn; e += MQE_NT) {
-                    const int ch = e / (tn * wn), r = e % (tn * wn), a = r % tn, b = r / tn;
-                    const int sp = ch >> 1, s = ch & 1;
-                    A[e] = G[(size_t)sp * tn + (size_t)s * wn * 2 * tn + a + (size_t)2 * tn * b];
-                }
-                __syncthreads();
+                compute_G_channels(C, i, A);
                 for (int kk = i + 1; kk < k; ++kk) {
                     carry_channels(C, kk, A, B);
                     T *t = A; A = B; B = t;
@@ -592,13 +715,13 @@ __device__ __forceinline__ double energy_sweep(const Ctx<T> &C, double *O_out, d
     return etot;
 }
 
-template <typename T>
-__device__ __forceinline__ void apply_field_everywhere(const Ctx<T> &C) {
+template <typename T, bool CH>
+__device__ __forceinline__ void apply_field_everywhere(const Ctx<T, CH> &C) {
     const double O[4] = {C.S->ssp[0], 0.0, 0.0, C.S->ssp[1]};
     for (int k = 0; k < C.S->L; ++k) apply_op(C, k, O);
 }
 
-template <typename T>
+template <typename T, bool CH>
 __global__ __launch_bounds__(MQE_NT, 2) void walker_kernel(Sys S, Pool P, const T *trial, int flags) {
     const int n = *P.n;
     if ((int)blockIdx.x >= n) return;
@@ -627,10 +750,11 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker_kernel(Sys S, Pool P, const 
     }
     __syncthreads();
 
-    Ctx<T> C;
+    Ctx<T, CH> C;
     C.tile = sh_tile;
     C.S = &S;
     C.trial = trial;
+    C.trialT = S.trialT;
     C.phi = (T *)P.phi + (size_t)slot * P.wflat;
     C.Lenv = (T *)P.Lenv + (size_t)slot * P.eflat;
     C.Renv = (T *)P.Renv + (size_t)slot * P.eflat;
@@ -1348,6 +1472,8 @@ struct mqe_engine {
     bool fp32 = false;
     bool tile32 = false;   // FP32 MFMA path for NN chains with bonds <= 32, h = 0
     float *d_t32N = nullptr, *d_t32T = nullptr;
+    float *d_trialT = nullptr;  // chained general kernel: every T_k[s] transposed
+    bool chained = false;       // the general kernel with chained FP32 products (bonds <= 64)
     hipStream_t stream = nullptr;
     std::vector<int> tvd, wmax;
     std::vector<long> toff, woff, eoff, moff;
@@ -1472,10 +1598,10 @@ static Book make_book(mqe_engine *e) {
     return B;
 }
 
-template <typename T>
+template <typename T, bool CH>
 static void launch_walker(mqe_engine *e, const Pool &P, int flags) {
-    hipLaunchKernelGGL(walker_kernel<T>, dim3(e->p.capacity), dim3(MQE_NT), e->gen_lds_bytes, e->stream, e->sys, P,
-                       (const T *)e->d_trial, flags);
+    hipLaunchKernelGGL((walker_kernel<T, CH>), dim3(e->p.capacity), dim3(MQE_NT), e->gen_lds_bytes, e->stream, e->sys,
+                       P, (const T *)e->d_trial, flags);
 }
 
 // Copy a control block value to the device with a kernel (no memset or copy
@@ -1525,8 +1651,9 @@ static int launch_walker_any(mqe_engine *e, const double *u, int flags) {
             hipLaunchKernelGGL(advance_samples_kernel, dim3(1), dim3(64), 0, e->stream, e->d_n, e->nC, e->d_nsamp,
                                e->max_samples);
         }
-    } else if (e->fp32) launch_walker<float>(e, P, flags);
-    else launch_walker<double>(e, P, flags);
+    } else if (e->chained) launch_walker<float, true>(e, P, flags);
+    else if (e->fp32) launch_walker<float, false>(e, P, flags);
+    else launch_walker<double, false>(e, P, flags);
     HIPCHK(hipGetLastError());
     return MQE_OK;
 }
@@ -1783,6 +1910,25 @@ int mqe_create(const mqe_params *p, const mqe_mps *trial, int n_couplings, const
                         : upload_mps_typed<double>(trial->data, tsize, &e->d_trial);
         if (r) return fail(r);
     }
+    {
+        const char *noch = getenv("MQE_NO_CHAIN");
+        e->chained = e->fp32 && !e->tile32 && e->Dt <= 64 && e->Dw <= 64 && !(noch && noch[0] == '1');
+    }
+    if (e->chained) {
+        // the trial with every T_k[s] transposed (t_{k+1} x t_k), same offsets:
+        // the chained products read trial operands along the inner index
+        std::vector<float> tt((size_t)tsize);
+        for (int k = 0; k < L; ++k) {
+            const int dl = e->tvd[k], dr = e->tvd[k + 1];
+            for (int s = 0; s < 2; ++s)
+                for (int c = 0; c < dr; ++c)
+                    for (int r = 0; r < dl; ++r)
+                        tt[e->toff[k] + (long)dl * dr * s + c + (long)dr * r] =
+                            (float)trial->data[e->toff[k] + r + (long)dl * (c + (long)dr * s)];
+        }
+        CK(hipMalloc((void **)&e->d_trialT, sizeof(float) * std::max<size_t>(tt.size(), 1)));
+        CK(hipMemcpy(e->d_trialT, tt.data(), sizeof(float) * tt.size(), hipMemcpyHostToDevice));
+    }
     if (e->tile32) {
         // trial in zero-padded 32x32 blocks, plain and transposed
         std::vector<float> tn((size_t)2 * L * 1024, 0.f), tt((size_t)2 * L * 1024, 0.f);
@@ -1818,10 +1964,12 @@ int mqe_create(const mqe_params *p, const mqe_mps *trial, int n_couplings, const
         const size_t need = std::min(sizeof(double) * ((size_t)2 * e->Dw * e->Dw + e->Dw), (size_t)72 * 1024);
         if (need >= sizeof(double) * (size_t)(e->Dw + 2 * e->Dw)) {
             e->gen_lds_bytes = need;
-            CK(hipFuncSetAttribute((const void *)walker_kernel<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)need));
-            CK(hipFuncSetAttribute((const void *)walker_kernel<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)need));
+            CK(hipFuncSetAttribute((const void *)walker_kernel<float, true>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+            CK(hipFuncSetAttribute((const void *)walker_kernel<float, false>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+            CK(hipFuncSetAttribute((const void *)walker_kernel<double, false>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
         }
     }
     const int nC = n_couplings;
@@ -1922,6 +2070,7 @@ int mqe_create(const mqe_params *p, const mqe_mps *trial, int n_couplings, const
     S.grid = e->d_grid;
     S.cs_ptr = e->d_csptr;
     S.cs_idx = e->d_csidx;
+    S.trialT = e->d_trialT;
     // S.S two-site tensor: Sx Sx - iSy iSy + Sz Sz (OpSx.cpp, OpISy.cpp, OpSz.cpp, d = 2)
     const double sx[4] = {0.0, 0.5, 0.5, 0.0}, isy[4] = {0.0, 0.5, -0.5, 0.0}, sz[4] = {-0.5, 0.0, 0.0, 0.5};
     for (int sp = 0; sp < 2; ++sp)
@@ -1952,7 +2101,7 @@ void mqe_destroy(mqe_engine *e) {
     if (!e) return;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
-    void *ptrs[] = {e->d_t32N, e->d_t32T, e->d_trial, e->d_tvd, e->d_first, e->d_second, e->d_nextfirst, e->d_csptr, e->d_csidx,
+    void *ptrs[] = {e->d_t32N, e->d_t32T, e->d_trialT, e->d_trial, e->d_tvd, e->d_first, e->d_second, e->d_nextfirst, e->d_csptr, e->d_csidx,
                     e->d_toff, e->d_woff, e->d_eoff, e->d_moff, e->d_Jc, e->d_S, e->d_U, e->d_VT, e->d_grid, e->d_phi,
                     e->d_Renv, e->d_Lenv, e->d_wvd, e->d_w, e->d_O, e->d_eh, e->d_EL, e->d_order, e->d_order2,
                     e->d_n, e->d_idx, e->d_u, e->d_scratch, e->d_status, e->d_sums, e->d_trace, e->d_ncopy,
