@@ -472,28 +472,30 @@ __device__ __forceinline__ void gemm(int m, int n, int k, const T *A, int lda, c
 
 // ---------------------------------------------------------------------------
 // Two chained FP32 matrix-core products with the intermediate in registers:
-//   F_f = A op(B_f)            (m1 x n1, inner dimension k1; f < NF, NF = 1, 2)
-//   D   = sum_f op(C_f) F_f    (mD x n1, inner dimension m1 <= 64, mD <= 64)
-// for nu independent units (operands of unit u from get(u)), whole workgroup,
-// no barrier inside.  A wave takes (unit, 32-column strip) items: for each f
-// it forms the strip of F_f in two 32 x 32 accumulator tiles (rows >= m1
-// zeroed) and feeds the accumulator slots straight into the second chain as
-// its B operand, accumulating the (at most two) row tiles of the D strip in
-// registers.  The second chain takes its inner index in the order the
-// accumulators hold it: MFMA q of F row tile t uses rows 32t + 8(q/4) + 4h +
-// q%4 (h = lane half), exactly accumulator slot q of lane (column, h), so no
-// lane exchange is needed; op(C) is loaded in the same order.
+//   F_f = A op(B_f)                      (m1 x n1, inner dimension k1)
+//   ND = 1:  D_0 = sum_f op(C_f) F_f     (f < NF, NF = 1, 2)
+//   ND = 2:  D_d = op(C_d) F_0           (d < 2, NF = 1: two outputs of one F)
+// (D_d: mD x n1; inner dimension m1 <= 64, mD <= 64) for nu independent units
+// (operands of unit u from get(u)), whole workgroup, no barrier inside.  A
+// wave takes (unit, 32-column strip) items: for each f it forms the strip of
+// F_f in two 32 x 32 accumulator tiles (rows >= m1 zeroed) and feeds the
+// accumulator slots straight into the second chain as its B operand,
+// accumulating the (at most two) row tiles of each D strip in registers.  The
+// second chain takes its inner index in the order the accumulators hold it:
+// MFMA q of F row tile t uses rows 32t + 8(q/4) + 4h + q%4 (h = lane half),
+// exactly accumulator slot q of lane (column, h), so no lane exchange is
+// needed; op(C) is loaded in the same order.
 // Items go to waves round-robin starting at wave0; the return value is the
-// wave that takes the next item, so several independent calls can share the
-// waves between two barriers.
-// TB: B_f stored n1 x k1 (op(B) = B^T).  TC: C_f stored m1 x mD (op(C) = C^T,
+// wave that takes the next item, so independent calls share the waves
+// between two barriers.
+// TB: B_f stored n1 x k1 (op(B) = B^T).  TC: C stored m1 x mD (op(C) = C^T,
 // contiguous along the inner index), else mD x m1.  All matrices column-major.
 // ---------------------------------------------------------------------------
 struct ChainOps {
     const float *A;      // m1 x k1, leading dimension lda
     const float *B[2];   // B_f
-    const float *C[2];   // C_f
-    float *D;            // mD x n1, leading dimension ldd
+    const float *C[2];   // C_f (ND = 1) or C_d (ND = 2)
+    float *D[2];         // D_d, mD x n1, leading dimension ldd
 };
 
 typedef float v16f_c __attribute__((ext_vector_type(16)));
@@ -525,30 +527,33 @@ __device__ __forceinline__ void buf_ld4(__amdgpu_buffer_rsrc_t r, int voff, floa
     v[3] = x.w;
 }
 
-template <bool TB, bool TC, int NF, class Get>
+// qsplit: the two row tiles of D are separate items (the F strips formed for
+// each), for calls with too few items to keep the four waves busy.
+template <bool TB, bool TC, int NF, int ND, class Get>
 __device__ __forceinline__ int chain2(int wave0, int nu, int m1, int n1, int k1, int mD, int lda, int ldb, int ldc,
-                                      int ldd, Get get) {
+                                      int ldd, bool qsplit, Get get) {
+    static_assert(ND == 1 || NF == 1, "two outputs take one intermediate");
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int nstrip = (n1 + 31) >> 5, nitems = nu * nstrip;
     const bool two = m1 > 32;     // second row tile of F
     const bool twoD = mD > 32;    // second row tile of D
+    const int nq = qsplit && twoD ? 2 : 1;
+    const int nstrip = (n1 + 31) >> 5, nitems = nu * nstrip * nq;
     const int sa = 4 * lda, sb = TB ? 4 * ldb : 4;  // bytes per k step of A and op(B)
     const int kf = k1 & ~15;      // whole 16-deep slices
 #pragma nounroll
     for (int it = (w - wave0 + 2 * MQE_WAVES) % MQE_WAVES; it < nitems; it += MQE_WAVES) {
-        const int u = it / nstrip, c0 = (it - u * nstrip) * 32;
+        const int qi = it % nq, rest = it / nq;
+        const int u = rest / nstrip, c0 = (rest - u * nstrip) * 32;
+        const bool do0 = nq == 1 || qi == 0, do1 = twoD && (nq == 1 || qi == 1);  // D row tiles of this item
         const ChainOps op = get(u);
-        // the lane coordinates as opaque values inside the loop: the per-element
+        // the lane coordinates as opaque values inside the loops: the per-element
         // offsets below are then formed next to their loads instead of being
-        // hoisted out of the loop and held in registers throughout
+        // hoisted out of the loops and held in registers throughout
         int r = lane & 31, h = lane >> 5;
-        asm volatile("" : "+v"(r), "+v"(h));
-        v16f_c D0, D1;
-#pragma unroll
-        for (int e = 0; e < 16; ++e) D0[e] = D1[e] = 0.f;
+        v16f_c D0, D1;  // the (at most two) row tiles of the D strip being formed
 #pragma nounroll
         for (int f = 0; f < NF; ++f) {
-            asm volatile("" : "+v"(r), "+v"(h));  // (the same for the loop over f)
+            asm volatile("" : "+v"(r), "+v"(h));
             const int col = min(c0 + r, n1 - 1);
             const int vA0 = 4 * min(r, m1 - 1), vA1 = 4 * min(32 + r, m1 - 1);
             const int vB = TB ? 4 * col : 4 * ldb * col;
@@ -562,6 +567,7 @@ __device__ __forceinline__ int chain2(int wave0, int nu, int m1, int n1, int k1,
             auto slice = [&](int k0, bool tail) {
                 float a0[8], a1[8], bv[8];
                 const int kl = k0 + 8 * h;
+                // tail: k >= k1 reads 0 (offset beyond the range)
 #pragma unroll
                 for (int q = 0; q < 8; ++q) {
                     const bool ok = !tail || kl + q < k1;
@@ -590,50 +596,189 @@ __device__ __forceinline__ int chain2(int wave0, int nu, int m1, int n1, int k1,
                     if (32 + row >= m1) F1[e] = 0.f;
                 }
             }
-            // ---- second chain: D (+)= op(C_f) F_f, row tile by row tile ----
-            const auto rC = buf_rsrc(op.C[f]);
-            const bool vecC = TC && (m1 & 3) == 0 && (ldc & 3) == 0 && aligned16(op.C[f], op.C[f], 0, 0);
-            auto chunk = [&](int row, int t, float (&cv)[16]) {
-                if (TC && vecC) {
-                    // a run at or beyond m1 meets zero rows of F: any values of C do
+            // ---- second chain: D_d (+)= op(C) F_f, row tile by row tile; a D strip
+            // is stored once its last term is in ----
+#pragma nounroll
+            for (int d = 0; d < ND; ++d) {
+                asm volatile("" : "+v"(r), "+v"(h));
+                if (ND == 2 || f == 0) {
 #pragma unroll
-                    for (int g = 0; g < 4; ++g)
-                        buf_ld4(rC, 4 * (ldc * row + min(32 * t + 8 * g + 4 * h, m1 - 4)), cv + 4 * g);
-                } else {
+                    for (int e = 0; e < 16; ++e) D0[e] = D1[e] = 0.f;
+                }
+                const float *Cp = op.C[ND == 1 ? f : d];
+                const auto rC = buf_rsrc(Cp);
+                const bool vecC = TC && (m1 & 3) == 0 && (ldc & 3) == 0 && aligned16(Cp, Cp, 0, 0);
+                auto chunk = [&](int row, int t, float (&cv)[16]) {
+                    if (TC && vecC) {
+                        // a run at or beyond m1 meets zero rows of F: any values of C do
 #pragma unroll
-                    for (int e = 0; e < 16; ++e) {
-                        const int k = min(32 * t + 8 * (e >> 2) + 4 * h + (e & 3), m1 - 1);
-                        cv[e] = buf_ld(rC, TC ? 4 * (ldc * row + k) : 4 * (row + ldc * k));
+                        for (int g = 0; g < 4; ++g)
+                            buf_ld4(rC, 4 * (ldc * row + min(32 * t + 8 * g + 4 * h, m1 - 4)), cv + 4 * g);
+                    } else {
+#pragma unroll
+                        for (int e = 0; e < 16; ++e) {
+                            const int k = min(32 * t + 8 * (e >> 2) + 4 * h + (e & 3), m1 - 1);
+                            cv[e] = buf_ld(rC, TC ? 4 * (ldc * row + k) : 4 * (row + ldc * k));
+                        }
+                    }
+                };
+                auto rowtile = [&](int row, v16f_c &Dq) {
+                    float c0v[16], c1v[16];
+                    chunk(row, 0, c0v);
+                    if (two) chunk(row, 1, c1v);
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) Dq = mfma32(c0v[q], F0[q], Dq);
+                    if (two) {
+#pragma unroll
+                        for (int q = 0; q < 16; ++q) Dq = mfma32(c1v[q], F1[q], Dq);
+                    }
+                };
+                if (do0) rowtile(row0, D0);
+                if (do1) rowtile(row1, D1);
+                if (f == NF - 1) {
+                    // rows 8(e/4) + 4h + e%4 (+32) of column c0 + r; outside D the store is dropped
+                    const int vd = c0 + r < n1 ? 4 * (ldd * (c0 + r) + 4 * h) : BUF_OOB;
+                    const auto rD = buf_rsrc(op.D[d]);
+                    if (do0) {
+#pragma unroll
+                        for (int e = 0; e < 16; ++e) {
+                            const int rr = 8 * (e >> 2) + 4 * h + (e & 3), imm = 4 * (8 * (e >> 2) + (e & 3));
+                            buf_st(D0[e], rD, rr < mD ? vd : BUF_OOB, imm);
+                        }
+                    }
+                    if (do1) {
+#pragma unroll
+                        for (int e = 0; e < 16; ++e) {
+                            const int rr = 32 + 8 * (e >> 2) + 4 * h + (e & 3), imm = 4 * (32 + 8 * (e >> 2) + (e & 3));
+                            buf_st(D1[e], rD, rr < mD ? vd : BUF_OOB, imm);
+                        }
                     }
                 }
-            };
-            auto rowtile = [&](int row, v16f_c &Dq) {
-                float c0v[16], c1v[16];
-                chunk(row, 0, c0v);
-                if (two) chunk(row, 1, c1v);
+            }
+        }
+    }
+    return (wave0 + nitems) % MQE_WAVES;
+}
+
+// The same chained products with 16-column strips (v_mfma_f32_16x16x4f32):
+// F row tiles and D row tiles of 16 rows (at most four each), lane (c, g)
+// (c = lane % 16, g = lane / 16) takes k = k0 + 4g + q of a 16-deep slice, and
+// accumulator slot e of lane (c, g) holds row 4g + e of its tile, so MFMA e of
+// F tile t in the second chain takes rows 16t + 4g + e.  For walker bonds below
+// 32 (no half-empty strips, twice the items); wider bonds keep 32-column strips
+// (a bond of 48 in three 16-column strips re-reads A and C three times and
+// measured 20 % slower than two 32-column strips).
+// qsplit: the D row tiles 0-1 and 2-3 are separate items.
+typedef float v4f_c __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ v4f_c mfma16(float a, float b, v4f_c c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+template <bool TB, bool TC, int NF, int ND, class Get>
+__device__ __forceinline__ int chain2_16(int wave0, int nu, int m1, int n1, int k1, int mD, int lda, int ldb, int ldc,
+                                         int ldd, bool qsplit, Get get) {
+    static_assert(ND == 1 || NF == 1, "two outputs take one intermediate");
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int MT = (m1 + 15) >> 4, QT = (mD + 15) >> 4;  // F and D row tiles (<= 4)
+    const int nq = qsplit && QT > 2 ? 2 : 1;
+    const int nstrip = (n1 + 15) >> 4, nitems = nu * nstrip * nq;
+    const int sa = 4 * lda, sb = TB ? 4 * ldb : 4;  // bytes per k step of A and op(B)
+    const int kf = k1 & ~15;
+#pragma nounroll
+    for (int it = (w - wave0 + 2 * MQE_WAVES) % MQE_WAVES; it < nitems; it += MQE_WAVES) {
+        const int qi = it % nq, rest = it / nq;
+        const int u = rest / nstrip, c0 = (rest - u * nstrip) * 16;
+        const int q0 = nq == 1 ? 0 : 2 * qi, q1 = nq == 1 ? QT : min(QT, 2 * qi + 2);  // D row tiles of this item
+        const ChainOps op = get(u);
+        int c = lane & 15, g = lane >> 4;
+        v4f_c D[4];
+#pragma nounroll
+        for (int f = 0; f < NF; ++f) {
+            asm volatile("" : "+v"(c), "+v"(g));
+            const int col = min(c0 + c, n1 - 1);
+            const int vB = TB ? 4 * col : 4 * ldb * col;
+            // ---- first chain: the F_f strip (16 columns), up to four 16-row tiles ----
+            v4f_c F[4];
 #pragma unroll
-                for (int q = 0; q < 16; ++q) Dq = mfma32(c0v[q], F0[q], Dq);
-                if (two) {
+            for (int t = 0; t < 4; ++t) F[t] = v4f_c{0.f, 0.f, 0.f, 0.f};
+            const auto rA = buf_rsrc(op.A), rB = buf_rsrc(op.B[f]);
+            auto slice = [&](int k0, bool tail) {
+                const int kl = k0 + 4 * g;
+                float a[4][4], bv[4];
 #pragma unroll
-                    for (int q = 0; q < 16; ++q) Dq = mfma32(c1v[q], F1[q], Dq);
+                for (int q = 0; q < 4; ++q) {
+                    const bool ok = !tail || kl + q < k1;
+                    bv[q] = buf_ld(rB, ok ? vB + sb * (kl + q) : BUF_OOB);
+#pragma unroll
+                    for (int t = 0; t < 4; ++t)
+                        a[t][q] = t < MT ? buf_ld(rA, ok ? 4 * min(16 * t + c, m1 - 1) + sa * (kl + q) : BUF_OOB) : 0.f;
                 }
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+#pragma unroll
+                    for (int t = 0; t < 4; ++t)
+                        if (t < MT) F[t] = mfma16(a[t][q], bv[q], F[t]);
             };
-            rowtile(row0, D0);
-            if (twoD) rowtile(row1, D1);
-        }
-        // rows 32 qt + 8(e/4) + 4h + e%4 of column c0 + r; outside D the store is dropped
-        const auto rD = buf_rsrc(op.D);
-        const int vd = c0 + r < n1 ? 4 * (ldd * (c0 + r) + 4 * h) : BUF_OOB;
+#pragma nounroll
+            for (int k0 = 0; k0 < kf; k0 += 16) slice(k0, false);
+            if (kf < k1) slice(kf, true);
+            if (m1 & 15) {  // rows of F beyond m1 (clamped A rows) must not contribute
 #pragma unroll
-        for (int e = 0; e < 16; ++e) {
-            const int rr = 8 * (e >> 2) + 4 * h + (e & 3), imm = 4 * (8 * (e >> 2) + (e & 3));
-            buf_st(D0[e], rD, rr < mD ? vd : BUF_OOB, imm);
-        }
-        if (twoD) {
+                for (int t = 0; t < 4; ++t)
 #pragma unroll
-            for (int e = 0; e < 16; ++e) {
-                const int rr = 32 + 8 * (e >> 2) + 4 * h + (e & 3), imm = 4 * (32 + 8 * (e >> 2) + (e & 3));
-                buf_st(D1[e], rD, rr < mD ? vd : BUF_OOB, imm);
+                    for (int e = 0; e < 4; ++e)
+                        if (16 * t + 4 * g + e >= m1) F[t][e] = 0.f;
+            }
+            // ---- second chain: D_d (+)= op(C) F_f, 16-row tile by tile ----
+#pragma nounroll
+            for (int d = 0; d < ND; ++d) {
+                asm volatile("" : "+v"(c), "+v"(g));
+                const float *Cp = op.C[ND == 1 ? f : d];
+                const auto rC = buf_rsrc(Cp);
+                const bool vecC = TC && (m1 & 3) == 0 && (ldc & 3) == 0 && aligned16(Cp, Cp, 0, 0);
+                if (ND == 2 || f == 0) {
+#pragma unroll
+                    for (int qd = 0; qd < 4; ++qd) D[qd] = v4f_c{0.f, 0.f, 0.f, 0.f};
+                }
+#pragma unroll
+                for (int qd = 0; qd < 4; ++qd) {
+                    if (qd < q0 || qd >= q1) continue;
+                    const int row = min(16 * qd + c, mD - 1);
+                    float cv[4][4];
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        if (t >= MT) continue;
+                        if (TC && vecC) {
+                            // a run at or beyond m1 meets zero rows of F: any values of C do
+                            buf_ld4(rC, 4 * (ldc * row + min(16 * t + 4 * g, m1 - 4)), cv[t]);
+                        } else {
+#pragma unroll
+                            for (int e = 0; e < 4; ++e) {
+                                const int k = min(16 * t + 4 * g + e, m1 - 1);
+                                cv[t][e] = buf_ld(rC, TC ? 4 * (ldc * row + k) : 4 * (row + ldc * k));
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        if (t >= MT) continue;
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) D[qd] = mfma16(cv[t][e], F[t][e], D[qd]);
+                    }
+                }
+                if (f == NF - 1) {
+                    // row 16 qd + 4g + e of column c0 + c; outside D the store is dropped
+                    const int vd = c0 + c < n1 ? 4 * (ldd * (c0 + c) + 4 * g) : BUF_OOB;
+                    const auto rD = buf_rsrc(op.D[d]);
+#pragma unroll
+                    for (int qd = 0; qd < 4; ++qd) {
+                        if (qd < q0 || qd >= q1) continue;
+#pragma unroll
+                        for (int e = 0; e < 4; ++e)
+                            buf_st(D[qd][e], rD, 16 * qd + 4 * g + e < mD ? vd : BUF_OOB, 4 * (16 * qd + e));
+                    }
+                }
             }
         }
     }

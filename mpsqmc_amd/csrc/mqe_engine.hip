@@ -204,30 +204,76 @@ struct Ctx {
     __device__ T *Mk(int k) const { return Renv + moff[k]; }  // right norm environment (w_k x w_k)
 };
 
-// G(s', s) = T_i[s']^T L_i phi_i[s] (t_{i+1} x w_{i+1}) chained in registers into
-// out + s' ssp + s ss (ld ldo); false (nothing done) where the chained
-// products do not apply.
+// Chained-product issue functions (CH instance only): they start their items at
+// wave wave0, return the wave for the next independent call and leave the
+// workgroup barrier to the caller.
+//
+// G(s', s) = T_i[s']^T L_i phi_i[s] (t_{i+1} x w_{i+1}) into out + s' ssp + s ss
+// (ld ldo).  One intermediate L_i phi_i[s] feeds both s' when the walker bond
+// spans two 32-column strips (four items either way).
 template <typename T, bool CH>
-__device__ __forceinline__ bool compute_G_chained(const Ctx<T, CH> &C, int i, const T *Lenv_i, T *out, int ldo, long ssp,
-                                                  long ss) {
-    if constexpr (sizeof(T) == sizeof(float)) {
+__device__ __forceinline__ int issue_G(const Ctx<T, CH> &C, int i, const T *Lenv_i, T *out, int ldo, long ssp, long ss,
+                                       int wave0) {
+    if constexpr (CH) {
         const int ti = C.tv[i], tn = C.tv[i + 1], wi = C.wv[i], wn = C.wv[i + 1];
-        if constexpr (CH) {
-            const float *Ti = (const float *)C.tsite(i), *ph = (const float *)C.site(i);
-            chain2<false, true, 1>(0, 4, ti, wn, wi, tn, ti, wi, ti, ldo, [&](int u) {
-                const int sp = u >> 1, s = u & 1;
+        const float *Ti = (const float *)C.tsite(i), *ph = (const float *)C.site(i);
+        auto unit = [&](int u) {
+            const int sp = u >> 1, s = u & 1;
+            ChainOps o;
+            o.A = (const float *)Lenv_i;
+            o.B[0] = o.B[1] = ph + (size_t)s * wi * wn;
+            o.C[0] = o.C[1] = Ti + (size_t)sp * ti * tn;
+            o.D[0] = o.D[1] = (float *)out + (size_t)sp * ssp + (size_t)s * ss;
+            return o;
+        };
+        if (wn < 32) return chain2_16<false, true, 1, 1>(wave0, 4, ti, wn, wi, tn, ti, wi, ti, ldo, false, unit);
+        if (wn > 32)
+            return chain2<false, true, 1, 2>(wave0, 2, ti, wn, wi, tn, ti, wi, ti, ldo, false, [&](int s) {
                 ChainOps o;
                 o.A = (const float *)Lenv_i;
                 o.B[0] = o.B[1] = ph + (size_t)s * wi * wn;
-                o.C[0] = o.C[1] = Ti + (size_t)sp * ti * tn;
-                o.D = (float *)out + (size_t)sp * ssp + (size_t)s * ss;
+                o.C[0] = Ti;
+                o.C[1] = Ti + (size_t)ti * tn;
+                o.D[0] = (float *)out + (size_t)s * ss;
+                o.D[1] = (float *)out + (size_t)ssp + (size_t)s * ss;
                 return o;
             });
-            __syncthreads();
-            return true;
-        }
+        return chain2<false, true, 1, 1>(wave0, 4, ti, wn, wi, tn, ti, wi, ti, ldo, false, unit);
     }
-    return false;
+    return wave0;
+}
+
+// V(t', t) = T_j[t'] R_{j+1} phi_j[t]^T (t_j x w_j) into the V scratch (see compute_V).
+template <typename T, bool CH>
+__device__ __forceinline__ int issue_V(const Ctx<T, CH> &C, int j, int wave0) {
+    if constexpr (CH) {
+        const int tj = C.tv[j], tn = C.tv[j + 1], wj = C.wv[j], wn = C.wv[j + 1];
+        const float *R = (const float *)C.Rk(j + 1), *ph = (const float *)C.site(j), *TTj = C.tsiteT(j);
+        float *V = (float *)(C.ts + C.sl.V);
+        auto unit = [&](int u) {
+            const int tp = u >> 1, t = u & 1;
+            ChainOps o;
+            o.A = R;
+            o.B[0] = o.B[1] = ph + (size_t)t * wj * wn;
+            o.C[0] = o.C[1] = TTj + (size_t)tp * tj * tn;
+            o.D[0] = o.D[1] = V + (size_t)tp * tj * 2 * wj + (size_t)t * tj * wj;
+            return o;
+        };
+        if (wj < 32) return chain2_16<true, true, 1, 1>(wave0, 4, tn, wj, wn, tj, tn, wj, tn, tj, false, unit);
+        if (wj > 32)
+            return chain2<true, true, 1, 2>(wave0, 2, tn, wj, wn, tj, tn, wj, tn, tj, false, [&](int t) {
+                ChainOps o;
+                o.A = R;
+                o.B[0] = o.B[1] = ph + (size_t)t * wj * wn;
+                o.C[0] = TTj;
+                o.C[1] = TTj + (size_t)tj * tn;
+                o.D[0] = V + (size_t)t * tj * wj;
+                o.D[1] = V + (size_t)tj * 2 * wj + (size_t)t * tj * wj;
+                return o;
+            });
+        return chain2<true, true, 1, 1>(wave0, 4, tn, wj, wn, tj, tn, wj, tn, tj, false, unit);
+    }
+    return wave0;
 }
 
 // Y = L_i phi_i (t_i x 2 w_{i+1});  G = T_i^T Y (2 t_{i+1} x 2 w_{i+1}), block
@@ -236,7 +282,11 @@ template <typename T, bool CH>
 __device__ __forceinline__ void compute_G(const Ctx<T, CH> &C, int i, const T *Lenv_i) {
     const int ti = C.tv[i], tn = C.tv[i + 1], wi = C.wv[i], wn = C.wv[i + 1];
     T *Y = C.ts + C.sl.Y, *G = C.ts + C.sl.G;
-    if (compute_G_chained(C, i, Lenv_i, G, 2 * tn, tn, (long)wn * 2 * tn)) return;
+    if constexpr (CH) {
+        issue_G(C, i, Lenv_i, G, 2 * tn, tn, (long)wn * 2 * tn, 0);
+        __syncthreads();
+        return;
+    }
     gemm<false, false, T>(ti, 2 * wn, wi, Lenv_i, ti, C.site(i), wi, Y, ti, false, C.tile);
     __syncthreads();
     gemm<true, false, T>(2 * tn, 2 * wn, ti, C.tsite(i), ti, Y, ti, G, 2 * tn, false, C.tile);
@@ -244,11 +294,12 @@ __device__ __forceinline__ void compute_G(const Ctx<T, CH> &C, int i, const T *L
 }
 
 // The four blocks G(s', s) of bond i+1 as contiguous channels ch = 2 s' + s
-// (t_{i+1} x w_{i+1}, ld t_{i+1}) at A.
+// (t_{i+1} x w_{i+1}, ld t_{i+1}) at A.  CH: no barrier at the end (the
+// caller's), the return value is the wave for the next independent call.
 template <typename T, bool CH>
-__device__ __forceinline__ void compute_G_channels(const Ctx<T, CH> &C, int i, T *A) {
+__device__ __forceinline__ int compute_G_channels(const Ctx<T, CH> &C, int i, T *A, int wave0 = 0) {
     const int tn = C.tv[i + 1], wn = C.wv[i + 1];
-    if (compute_G_chained(C, i, C.Lk(i), A, tn, 2L * tn * wn, (long)tn * wn)) return;
+    if constexpr (CH) return issue_G(C, i, C.Lk(i), A, tn, 2L * tn * wn, (long)tn * wn, wave0);
     compute_G(C, i, C.Lk(i));
     const T *G = C.ts + C.sl.G;
     for (int e = threadIdx.x; e < 4 * tn * wn; e += MQE_NT) {
@@ -257,31 +308,21 @@ __device__ __forceinline__ void compute_G_channels(const Ctx<T, CH> &C, int i, T
         A[e] = G[(size_t)sp * tn + (size_t)s * wn * 2 * tn + a + (size_t)2 * tn * b];
     }
     __syncthreads();
+    return wave0;
 }
 
 // W[t] = R_{j+1} phi_j[t]^T (t_{j+1} x w_j);  V[t'] = T_j[t'] W (t_j x 2 w_j);
 // block (t', t) at V + t' (t_j 2 w_j) + t (t_j w_j), ld t_j.
 template <typename T, bool CH>
 __device__ __forceinline__ void compute_V(const Ctx<T, CH> &C, int j) {
+    if constexpr (CH) {
+        issue_V(C, j, 0);
+        __syncthreads();
+        return;
+    }
     const int tj = C.tv[j], tn = C.tv[j + 1], wj = C.wv[j], wn = C.wv[j + 1];
     T *W = C.ts + C.sl.W, *V = C.ts + C.sl.V;
     const T *R = C.Rk(j + 1), *ph = C.site(j);
-    if constexpr (sizeof(T) == sizeof(float)) {
-        if constexpr (CH) {
-            const float *TTj = C.tsiteT(j);
-            chain2<true, true, 1>(0, 4, tn, wj, wn, tj, tn, wj, tn, tj, [&](int u) {
-                const int tp = u >> 1, t = u & 1;
-                ChainOps o;
-                o.A = (const float *)R;
-                o.B[0] = o.B[1] = (const float *)ph + (size_t)t * wj * wn;
-                o.C[0] = o.C[1] = TTj + (size_t)tp * tj * tn;
-                o.D = (float *)V + (size_t)tp * tj * 2 * wj + (size_t)t * tj * wj;
-                return o;
-            });
-            __syncthreads();
-            return;
-        }
-    }
     gemm_batch<false, true, T>(2, tn, wj, wn, R, tn, 0, ph, wj, (long)wj * wn, W, tn, (long)tn * wj, false, C.tile);
     __syncthreads();
     const T *Tj = C.tsite(j);
@@ -299,16 +340,20 @@ __device__ __forceinline__ void carry_channels(const Ctx<T, CH> &C, int k, const
     if constexpr (sizeof(T) == sizeof(float)) {
         if constexpr (CH) {
             const float *ph = (const float *)C.site(k);
-            chain2<false, true, 2>(0, 4, tk, wn, wk, tn, tk, wk, tk, tn, [&](int ch) {
+            auto unit = [&](int ch) {
                 ChainOps o;
                 o.A = (const float *)src + (size_t)ch * tk * wk;
                 o.B[0] = ph;
                 o.B[1] = ph + (size_t)wk * wn;
                 o.C[0] = (const float *)Tk;
                 o.C[1] = (const float *)Tk + (size_t)tk * tn;
-                o.D = (float *)dst + (size_t)ch * tn * wn;
+                o.D[0] = o.D[1] = (float *)dst + (size_t)ch * tn * wn;
                 return o;
-            });
+            };
+            if (wn < 32)
+                chain2_16<false, true, 2, 1>(0, 4, tk, wn, wk, tn, tk, wk, tk, tn, false, unit);
+            else
+                chain2<false, true, 2, 1>(0, 4, tk, wn, wk, tn, tk, wk, tk, tn, false, unit);
             __syncthreads();
             return;
         }
@@ -326,7 +371,9 @@ __device__ __forceinline__ void carry_channels(const Ctx<T, CH> &C, int k, const
 // Two-site mixed tensor M[(s'+2t')*4 + (s+2t)] = <G(s',s), V(t',t)> at a bond
 // with trial dim tb and walker dim wb.  g[ch], ch = 2 s' + s, with leading dim gld.
 template <typename T, bool CH>
-__device__ __forceinline__ void pair_tensor(const Ctx<T, CH> &C, int tb, int wb, const T *const g[4], int gld, double *M) {
+// Rout (or nullptr): also writes the right environment V(0,0) + V(1,1) there.
+__device__ __forceinline__ void pair_tensor(const Ctx<T, CH> &C, int tb, int wb, const T *const g[4], int gld, double *M,
+                                            T *Rout = nullptr) {
     const T *V = C.ts + C.sl.V;
     double acc[16];
 #pragma unroll
@@ -335,13 +382,16 @@ __device__ __forceinline__ void pair_tensor(const Ctx<T, CH> &C, int tb, int wb,
     for (int e = threadIdx.x; e < nel; e += MQE_NT) {
         const int a = e % tb, b = e / tb;
         double gv[4], vv[4];
+        T vt[4];
 #pragma unroll
         for (int ch = 0; ch < 4; ++ch) gv[ch] = (double)g[ch][a + (size_t)gld * b];
 #pragma unroll
         for (int ch = 0; ch < 4; ++ch) {
             const int tp = ch >> 1, t = ch & 1;
-            vv[ch] = (double)V[(size_t)tp * tb * 2 * wb + (size_t)t * tb * wb + a + (size_t)tb * b];
+            vt[ch] = V[(size_t)tp * tb * 2 * wb + (size_t)t * tb * wb + a + (size_t)tb * b];
+            vv[ch] = (double)vt[ch];
         }
+        if (Rout) Rout[e] = vt[0] + vt[3];
 #pragma unroll
         for (int sp = 0; sp < 2; ++sp)
 #pragma unroll
@@ -366,8 +416,14 @@ __device__ __forceinline__ void pair_for_coupling(const Ctx<T, CH> &C, int i, in
     const T *g[4];
     int gld;
     if (j == i + 1) {
-        compute_G(C, i, C.Lk(i));
-        const T *G = C.ts + C.sl.G;
+        T *G = C.ts + C.sl.G;
+        if constexpr (CH) {  // G and V in one barrier interval
+            issue_V(C, j, issue_G(C, i, C.Lk(i), G, 2 * tn, tn, (long)wn * 2 * tn, 0));
+            __syncthreads();
+        } else {
+            compute_G(C, i, C.Lk(i));
+            compute_V(C, j);
+        }
         for (int ch = 0; ch < 4; ++ch) {
             const int sp = ch >> 1, s = ch & 1;
             g[ch] = G + (size_t)sp * tn + (size_t)s * wn * 2 * tn;
@@ -376,7 +432,12 @@ __device__ __forceinline__ void pair_for_coupling(const Ctx<T, CH> &C, int i, in
     } else {
         // the four blocks as contiguous channels (ch = 2 s' + s), carried to bond j
         T *A = C.ts + C.sl.GcA, *B = C.ts + C.sl.GcB;
-        compute_G_channels(C, i, A);
+        if constexpr (CH) {  // G channels and V in one barrier interval
+            issue_V(C, j, compute_G_channels(C, i, A));
+            __syncthreads();
+        } else {
+            compute_G_channels(C, i, A);
+        }
         for (int k = i + 1; k < j; ++k) {
             carry_channels(C, k, A, B);
             T *t = A; A = B; B = t;
@@ -384,8 +445,8 @@ __device__ __forceinline__ void pair_for_coupling(const Ctx<T, CH> &C, int i, in
         const int tb = C.tv[j], wb = C.wv[j];
         for (int ch = 0; ch < 4; ++ch) g[ch] = A + (size_t)ch * tb * wb;
         gld = tb;
+        if constexpr (!CH) compute_V(C, j);
     }
-    compute_V(C, j);
     pair_tensor(C, C.tv[j], C.wv[j], g, gld, M);
 }
 
@@ -396,16 +457,20 @@ __device__ __forceinline__ void advance_L(const Ctx<T, CH> &C, int k) {
     if constexpr (sizeof(T) == sizeof(float)) {
         if constexpr (CH) {
             const float *ph = (const float *)C.site(k), *Tk = (const float *)C.tsite(k);
-            chain2<false, true, 2>(0, 1, tk, wn, wk, tn, tk, wk, tk, tn, [&](int) {
+            auto unit = [&](int) {
                 ChainOps o;
                 o.A = (const float *)C.Lk(k);
                 o.B[0] = ph;
                 o.B[1] = ph + (size_t)wk * wn;
                 o.C[0] = Tk;
                 o.C[1] = Tk + (size_t)tk * tn;
-                o.D = (float *)C.Lk(k + 1);
+                o.D[0] = o.D[1] = (float *)C.Lk(k + 1);
                 return o;
-            });
+            };
+            if (wn < 32)
+                chain2_16<false, true, 2, 1>(0, 1, tk, wn, wk, tn, tk, wk, tk, tn, true, unit);
+            else
+                chain2<false, true, 2, 1>(0, 1, tk, wn, wk, tn, tk, wk, tk, tn, true, unit);
             __syncthreads();
             return;
         }
@@ -426,16 +491,20 @@ __device__ __forceinline__ void rebuild_R(const Ctx<T, CH> &C, int k) {
         const int tk = C.tv[k], tn = C.tv[k + 1], wk = C.wv[k], wn = C.wv[k + 1];
         if constexpr (CH) {
             const float *ph = (const float *)C.site(k), *TTk = C.tsiteT(k);
-            chain2<true, true, 2>(0, 1, tn, wk, wn, tk, tn, wk, tn, tk, [&](int) {
+            auto unit = [&](int) {
                 ChainOps o;
                 o.A = (const float *)C.Rk(k + 1);
                 o.B[0] = ph;
                 o.B[1] = ph + (size_t)wk * wn;
                 o.C[0] = TTk;
                 o.C[1] = TTk + (size_t)tk * tn;
-                o.D = (float *)C.Rk(k);
+                o.D[0] = o.D[1] = (float *)C.Rk(k);
                 return o;
-            });
+            };
+            if (wk < 32)
+                chain2_16<true, true, 2, 1>(0, 1, tn, wk, wn, tk, tn, wk, tn, tk, true, unit);
+            else
+                chain2<true, true, 2, 1>(0, 1, tn, wk, wn, tk, tn, wk, tn, tk, true, unit);
             __syncthreads();
             return;
         }
@@ -450,26 +519,37 @@ __device__ __forceinline__ void rebuild_R(const Ctx<T, CH> &C, int k) {
 
 // Right norm environment M_k = sum_s phi_k[s] M_{k+1} phi_k[s]^T (w_k x w_k):
 // the norm of the walker's part right of bond k in the current gauge.
+// (CH issue function: M_{k+1} is symmetric, M_k = sum_s phi_k[s] (M_{k+1} phi_k[s]^T))
+template <typename T, bool CH>
+__device__ __forceinline__ int issue_M(const Ctx<T, CH> &C, int k, int wave0) {
+    if constexpr (CH) {
+        const int wk = C.wv[k], wn = C.wv[k + 1];
+        const float *ph = (const float *)C.site(k), *Mn = (const float *)C.Mk(k + 1);
+        float *Mk = (float *)C.Mk(k);
+        auto unit = [&](int) {
+            ChainOps o;
+            o.A = Mn;
+            o.B[0] = o.C[0] = ph;
+            o.B[1] = o.C[1] = ph + (size_t)wk * wn;
+            o.D[0] = o.D[1] = Mk;
+            return o;
+        };
+        if (wk < 32) return chain2_16<true, false, 2, 1>(wave0, 1, wn, wk, wn, wk, wn, wk, wk, wk, false, unit);
+        return chain2<true, false, 2, 1>(wave0, 1, wn, wk, wn, wk, wn, wk, wk, wk, false, unit);
+    }
+    return wave0;
+}
+
 template <typename T, bool CH>
 __device__ __forceinline__ void advance_M(const Ctx<T, CH> &C, int k) {
     const int wk = C.wv[k], wn = C.wv[k + 1];
     T *X = C.ts + C.sl.Y;
     const T *ph = C.site(k), *Mn = C.Mk(k + 1);
     T *Mk = C.Mk(k);
-    if constexpr (sizeof(T) == sizeof(float)) {
-        if constexpr (CH) {
-            // M_{k+1} is symmetric: M_k = sum_s phi_k[s] (M_{k+1} phi_k[s]^T)
-            chain2<true, false, 2>(0, 1, wn, wk, wn, wk, wn, wk, wk, wk, [&](int) {
-                ChainOps o;
-                o.A = (const float *)Mn;
-                o.B[0] = o.C[0] = (const float *)ph;
-                o.B[1] = o.C[1] = (const float *)ph + (size_t)wk * wn;
-                o.D = (float *)Mk;
-                return o;
-            });
-            __syncthreads();
-            return;
-        }
+    if constexpr (CH) {
+        issue_M(C, k, 0);
+        __syncthreads();
+        return;
     }
     gemm_batch<false, false, T>(2, wk, wn, wn, ph, wk, (long)wk * wn, Mn, wn, 0, X, wk, (long)wk * wn, false, C.tile);
     __syncthreads();
@@ -620,12 +700,15 @@ __device__ __forceinline__ double energy_sweep(const Ctx<T, CH> &C, double *O_ou
     set_one(C.Mk(L));
     double etot = 0.0;
     for (int k = L - 1; k >= 1; --k) {
+        bool nn_ready = false;   // G of the coupling (k-1, k) already formed
+        bool r_pending = false;  // R_k still to be written (by the next pair tensor)
         if (canon) {
             lq_site(C, k);  // V of bond k now holds T_k R_{k+1} Q_k^T blocks
         } else {
             // light sweep: no re-orthonormalisation; site k is rescaled to the
-            // Frobenius norm sqrt(w_k) of a right-canonical site and the factor
-            // moved into site k-1 (the state is unchanged).  This keeps every
+            // Frobenius norm sqrt(w_k) of a right-canonical site when it is more
+            // than a factor 4 away from it, and the factor moved into site k-1
+            // (the state is unchanged).  This keeps every
             // tensor, and the environments built from them, at the magnitudes of
             // the canonical form: a unit-norm target would shrink R_k by about
             // sqrt(w) per site and underflow FP32 on long chains at large D.
@@ -636,18 +719,38 @@ __device__ __forceinline__ double energy_sweep(const Ctx<T, CH> &C, double *O_ou
                 double s2 = 0.0;
                 for (int e = threadIdx.x; e < 2 * wk * wn; e += MQE_NT) s2 += (double)ph[e] * (double)ph[e];
                 const double f = sqrt(block_sum(s2, C.red) / (double)wk);
-                if (f > 0.0 && isfinite(f)) {
+                // only a site that has drifted by more than a factor 4 is rescaled:
+                // that bounds every magnitude just as well and skips the two
+                // passes over the tensors at most sites of most steps
+                if (f > 0.0 && isfinite(f) && (f > 4.0 || f < 0.25)) {
                     scale_site(C, k, 1.0 / f);
                     scale_site(C, k - 1, f);
                 }
             }
-            compute_V(C, k);
-            const int tk = C.tv[k], wk = C.wv[k];
-            const T *V = C.ts + C.sl.V;
-            T *Rk = C.Rk(k);
-            for (int e = threadIdx.x; e < tk * wk; e += MQE_NT)
-                Rk[e] = V[e] + V[(size_t)tk * 2 * wk + (size_t)tk * wk + e];
-            advance_M(C, k);
+            if constexpr (CH) {
+                // V, M_k and the nearest-neighbour G of bond k (the last coupling
+                // ending at k, if it is (k-1, k)) in one barrier interval
+                const int wq = issue_M(C, k, issue_V(C, k, 0));
+                const int ql = S.cs_ptr[k + 1] - 1;
+                if (ql >= S.cs_ptr[k] && S.first[S.cs_idx[ql]] == k - 1) {
+                    const int tn = C.tv[k], wn = C.wv[k];
+                    issue_G(C, k - 1, C.Lk(k - 1), C.ts + C.sl.G, 2 * tn, tn, (long)wn * 2 * tn, wq);
+                    nn_ready = true;
+                }
+                __syncthreads();
+            } else {
+                compute_V(C, k);
+            }
+            if (S.cs_ptr[k + 1] > S.cs_ptr[k]) {
+                r_pending = true;  // R_k from the first pair tensor of the site
+            } else {
+                const int tk = C.tv[k], wk = C.wv[k];
+                const T *V = C.ts + C.sl.V;
+                T *Rk = C.Rk(k);
+                for (int e = threadIdx.x; e < tk * wk; e += MQE_NT)
+                    Rk[e] = V[e] + V[(size_t)tk * 2 * wk + (size_t)tk * wk + e];
+            }
+            if constexpr (!CH) advance_M(C, k);
         }
         for (int q = S.cs_ptr[k]; q < S.cs_ptr[k + 1]; ++q) {
             const int c = S.cs_idx[q], i = S.first[c];
@@ -655,7 +758,7 @@ __device__ __forceinline__ double energy_sweep(const Ctx<T, CH> &C, double *O_ou
             const T *g[4];
             int gld;
             if (i == k - 1) {
-                compute_G(C, i, C.Lk(i));
+                if (!nn_ready) compute_G(C, i, C.Lk(i));
                 const T *G = C.ts + C.sl.G;
                 for (int ch = 0; ch < 4; ++ch) {
                     const int sp = ch >> 1, s = ch & 1;
@@ -665,6 +768,7 @@ __device__ __forceinline__ double energy_sweep(const Ctx<T, CH> &C, double *O_ou
             } else {
                 T *A = C.ts + C.sl.GcA, *B = C.ts + C.sl.GcB;
                 compute_G_channels(C, i, A);
+                if constexpr (CH) __syncthreads();
                 for (int kk = i + 1; kk < k; ++kk) {
                     carry_channels(C, kk, A, B);
                     T *t = A; A = B; B = t;
@@ -672,7 +776,8 @@ __device__ __forceinline__ double energy_sweep(const Ctx<T, CH> &C, double *O_ou
                 for (int ch = 0; ch < 4; ++ch) g[ch] = A + (size_t)ch * C.tv[k] * C.wv[k];
                 gld = C.tv[k];
             }
-            pair_tensor(C, C.tv[k], C.wv[k], g, gld, M);
+            pair_tensor(C, C.tv[k], C.wv[k], g, gld, M, r_pending ? C.Rk(k) : nullptr);
+            r_pending = false;
             double e = 0.0;
             for (int q2 = 0; q2 < 16; ++q2) e += S.SS[q2] * M[q2];
             etot += S.Jc[c] * e;
@@ -721,6 +826,85 @@ __device__ __forceinline__ void apply_field_everywhere(const Ctx<T, CH> &C) {
     for (int k = 0; k < C.S->L; ++k) apply_op(C, k, O);
 }
 
+// combos, PDF over the grid, CDF sample and the sampled operators of one
+// coupling (MPSQMC2.cpp:540-588, MPSstate.cpp:589-631) for at most 256 grid
+// points: every wave computes the same result (lane q < 16 forms combo q, lane
+// l owns grid points 4l .. 4l+3; wave-wide sums and scan), so no workgroup
+// barrier is needed.  M: the 16 mixed matrix elements (LDS).
+__device__ __forceinline__ void wave_sample(const double *M, const double *tS, const double *tU, const double *tVT,
+                                            const double *grid, int Ng, double O, double u, double &pdfnorm,
+                                            int &gs, double &fstar, double (&Ux)[4], double (&Vx)[4]) {
+    const int lane = threadIdx.x & 63;
+    double cq = 0.0;
+    if (lane < 16) {
+        // combos[a + 4b] = sqrt(s_a s_b) <trial| U_a (x) V_b |phi>
+        const int a = lane & 3, bb = lane >> 2;
+        double sacc = 0.0;
+        for (int sp = 0; sp < 2; ++sp)
+            for (int ss = 0; ss < 2; ++ss) {
+                const double ua = tU[sp + 2 * (ss + 2 * a)];
+                for (int tp = 0; tp < 2; ++tp)
+                    for (int t = 0; t < 2; ++t) sacc += ua * tVT[bb + 4 * (tp + 2 * t)] * M[(sp + 2 * tp) * 4 + (ss + 2 * t)];
+            }
+        cq = sqrt(tS[a] * tS[bb]) * sacc;
+    }
+    double cb[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) cb[q] = readlane_d(cq, q);
+    auto quad = [&](const double *x) {  // sum_{l,r} comb[l + 4r] x_l x_r
+        double f = 0.0;
+#pragma unroll
+        for (int l = 0; l < 4; ++l) {
+            double t = 0.0;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) t += cb[l + 4 * r] * x[r];
+            f += x[l] * t;
+        }
+        return f;
+    };
+    const double pref = 4.0 / (O * Ng);
+    double pv[4], loc = 0.0;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int g = 4 * lane + t;
+        double v = 0.0;
+        if (g < Ng) {
+            const double f = quad(grid + 4 * g);
+            v = (f > 0.0 ? f : 0.0) * pref;
+        }
+        pv[t] = v;
+        loc += v;
+    }
+    pdfnorm = wave_sum(loc);
+    const double inv = 1.0 / pdfnorm;
+    double ln = 0.0;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        pv[t] *= inv;
+        ln += pv[t];
+    }
+    // count of the grid points g < Ng - 1 whose cumulative probability is <= u
+    double run = wave_exclusive_scan(ln), cnt = 0.0;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        run += pv[t];
+        if (4 * lane + t < Ng - 1 && run <= u) cnt += 1.0;
+    }
+    gs = (int)wave_sum(cnt);
+    const double *xs = grid + 4 * gs;
+    fstar = quad(xs);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) Ux[q] = Vx[q] = 0.0;
+    for (int k = 0; k < 4; ++k) {
+        const double pf = sqrt(tS[k] * 4.0) * xs[k];  // sqrt(4 S_k) x_k (MPSstate.cpp:591,616)
+        for (int up = 0; up < 2; ++up)
+            for (int dn = 0; dn < 2; ++dn) {
+                Ux[up + 2 * dn] += pf * tU[up + 2 * (dn + 2 * k)];
+                Vx[up + 2 * dn] += pf * tVT[k + 4 * (up + 2 * dn)];
+            }
+    }
+}
+
 template <typename T, bool CH>
 __global__ __launch_bounds__(MQE_NT, 2) void walker_kernel(Sys S, Pool P, const T *trial, int flags) {
     const int n = *P.n;
@@ -733,9 +917,7 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker_kernel(Sys S, Pool P, const 
 
     __shared__ int tv[MQE_MAXL + 1], wv[MQE_MAXL + 1];
     __shared__ double red[MQE_NT + 16 * MQE_WAVES];
-    __shared__ double M[16], comb[16], Ux[4], Vx[4];
-    __shared__ double sh_scal[4];
-    __shared__ int sh_g;
+    __shared__ double M[16], comb[16];
 
     __shared__ int sh_tile;
     for (int k = threadIdx.x; k <= L; k += MQE_NT) {
@@ -808,62 +990,55 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker_kernel(Sys S, Pool P, const 
         }
         while (rv > j + 1) rebuild_R(C, --rv);
         pair_for_coupling(C, i, j, M);
-        // combos[a + 4b] = sqrt(s_a s_b) <trial| U_a (x) V_b |phi>  (MPSQMC2.cpp:540-549)
         const double *tS = S.tS + 4 * c, *tU = S.tU + 16 * c, *tVT = S.tVT + 16 * c;
-        if (threadIdx.x < 16) {
-            const int a = threadIdx.x & 3, bb = threadIdx.x >> 2;
-            double s = 0.0;
-            for (int sp = 0; sp < 2; ++sp)
-                for (int ss = 0; ss < 2; ++ss) {
-                    const double ua = tU[sp + 2 * (ss + 2 * a)];
-                    if (ua == 0.0) continue;
-                    for (int tp = 0; tp < 2; ++tp)
-                        for (int t = 0; t < 2; ++t)
-                            s += ua * tVT[bb + 4 * (tp + 2 * t)] * M[(sp + 2 * tp) * 4 + (ss + 2 * t)];
-                }
-            comb[threadIdx.x] = sqrt(tS[a] * tS[bb]) * s;
-        }
-        __syncthreads();
-        // PDF over the grid (MPSQMC2.cpp:552-574)
-        double *pdf = C.ds + C.sl.PDF, *cum = C.ds + C.sl.CUM;
-        const int Ng = S.Ng;
-        const double pref = 4.0 / (O * Ng);
-        double part = 0.0;
-        for (int g = threadIdx.x; g < Ng; g += MQE_NT) {
-            const double *x = S.grid + 4 * g;
-            double f = 0.0;
-            for (int l = 0; l < 4; ++l)
-                for (int r = 0; r < 4; ++r) f += comb[l + 4 * r] * x[l] * x[r];
-            double v = f > 0.0 ? f : 0.0;
-            v *= pref;
-            pdf[g] = v;
-            part += v;
-        }
-        const double pdfnorm = block_sum(part, red);
-        const double inv = 1.0 / pdfnorm;
-        for (int g = threadIdx.x; g < Ng; g += MQE_NT) pdf[g] *= inv;
-        __syncthreads();
-        block_inclusive_scan(Ng, pdf, cum, red);
         const double u = P.u ? P.u[(size_t)b * S.nC + c] : philox_uniform(P.seed, (uint32_t)b, (uint32_t)c, (uint32_t)ctl.step, 0u);
-        double cnt = 0.0;
-        for (int g = threadIdx.x; g < Ng - 1; g += MQE_NT) cnt += (cum[g] <= u) ? 1.0 : 0.0;
-        cnt = block_sum(cnt, red);
-        const int gs = (int)cnt;
-        w *= pdfnorm;
-        if (threadIdx.x == 0) {
-            sh_g = gs;
-            if (P.idx) P.idx[(size_t)b * S.nC + c] = gs;
-            if (!(pdfnorm > 0.0) || !isfinite(pdfnorm)) atomicOr(P.status, 1);
-            // sampled operators (MPSstate.cpp:589-631)
+        double pdfnorm, fstar, Ux[4], Vx[4];
+        int gs;
+        if (S.Ng <= 256) {
+            wave_sample(M, tS, tU, tVT, S.grid, S.Ng, O, u, pdfnorm, gs, fstar, Ux, Vx);
+        } else {
+            // larger grids: the workgroup forms the PDF and scans it (MPSQMC2.cpp:540-588)
+            if (threadIdx.x < 16) {
+                const int a = threadIdx.x & 3, bb = threadIdx.x >> 2;
+                double sacc = 0.0;
+                for (int sp = 0; sp < 2; ++sp)
+                    for (int ss = 0; ss < 2; ++ss) {
+                        const double ua = tU[sp + 2 * (ss + 2 * a)];
+                        for (int tp = 0; tp < 2; ++tp)
+                            for (int t = 0; t < 2; ++t)
+                                sacc += ua * tVT[bb + 4 * (tp + 2 * t)] * M[(sp + 2 * tp) * 4 + (ss + 2 * t)];
+                    }
+                comb[threadIdx.x] = sqrt(tS[a] * tS[bb]) * sacc;
+            }
+            __syncthreads();
+            double *pdf = C.ds + C.sl.PDF, *cum = C.ds + C.sl.CUM;
+            const int Ng = S.Ng;
+            const double pref = 4.0 / (O * Ng);
+            double part = 0.0;
+            for (int g = threadIdx.x; g < Ng; g += MQE_NT) {
+                const double *x = S.grid + 4 * g;
+                double f = 0.0;
+                for (int l = 0; l < 4; ++l)
+                    for (int r = 0; r < 4; ++r) f += comb[l + 4 * r] * x[l] * x[r];
+                const double v = (f > 0.0 ? f : 0.0) * pref;
+                pdf[g] = v;
+                part += v;
+            }
+            pdfnorm = block_sum(part, red);
+            const double inv = 1.0 / pdfnorm;
+            for (int g = threadIdx.x; g < Ng; g += MQE_NT) pdf[g] *= inv;
+            __syncthreads();
+            block_inclusive_scan(Ng, pdf, cum, red);
+            double cnt = 0.0;
+            for (int g = threadIdx.x; g < Ng - 1; g += MQE_NT) cnt += (cum[g] <= u) ? 1.0 : 0.0;
+            gs = (int)block_sum(cnt, red);
             const double *x = S.grid + 4 * gs;
-            double f = 0.0;
+            fstar = 0.0;
             for (int l = 0; l < 4; ++l)
-                for (int r = 0; r < 4; ++r) f += comb[l + 4 * r] * x[l] * x[r];
-            sh_scal[0] = f;
-            for (int q = 0; q < 4; ++q) { Ux[q] = 0.0; Vx[q] = 0.0; }
+                for (int r = 0; r < 4; ++r) fstar += comb[l + 4 * r] * x[l] * x[r];
+            for (int q = 0; q < 4; ++q) Ux[q] = Vx[q] = 0.0;
             for (int k = 0; k < 4; ++k) {
-                const double pf = sqrt(tS[k] * 4.0) * x[k];
-                if (pf == 0.0) continue;
+                const double pf = sqrt(tS[k] * 4.0) * x[k];  // MPSstate.cpp:591,616
                 for (int up = 0; up < 2; ++up)
                     for (int dn = 0; dn < 2; ++dn) {
                         Ux[up + 2 * dn] += pf * tU[up + 2 * (dn + 2 * k)];
@@ -871,31 +1046,48 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker_kernel(Sys S, Pool P, const 
                     }
             }
         }
-        __syncthreads();
-        // right site scaled by O / <trial|B(x) phi> = O / (4 f*): the overlap stays O
-        const double alpha = O / (4.0 * sh_scal[0]);
-        const double Va[4] = {alpha * Vx[0], alpha * Vx[1], alpha * Vx[2], alpha * Vx[3]};
-        apply_op(C, i, Ux);
-        apply_op(C, j, Va);
-        if (j == i + 1 && S.nextfirst[c] > i) {
-            // nearest neighbours, and the next coupling starts right of i: the
-            // left boundary moves to j, L_{i+1} = sum_{s,s0} Ux[s,s0] G(s, s0)
-            const int tn = tv[i + 1], wn = wv[i + 1];
-            const T *G = C.ts + C.sl.G;
-            T *Ln = C.Lk(i + 1);
+        w *= pdfnorm;
+        if (threadIdx.x == 0) {
+            if (P.idx) P.idx[(size_t)b * S.nC + c] = gs;
+            if (!(pdfnorm > 0.0) || !isfinite(pdfnorm)) atomicOr(P.status, 1);
+        }
+        // the sampled operators on sites i and j (MPSstate.cpp:575-633), the right
+        // one scaled by O / <trial|B(x) phi> = O / (4 f*) so that the overlap stays O;
+        // for nearest neighbours whose next coupling starts right of i, the left
+        // boundary moves to j: L_{i+1} = sum_{s,s0} Ux[s,s0] G(s, s0).  One pass.
+        const double alpha = O / (4.0 * fstar);
+        const bool nn_move = j == i + 1 && S.nextfirst[c] > i;
+        {
+            T *pi = C.site(i), *pj = C.site(j);
+            const int bi = wv[i] * wv[i + 1], bj = wv[j] * wv[j + 1];
             const T u00 = (T)Ux[0], u10 = (T)Ux[1], u01 = (T)Ux[2], u11 = (T)Ux[3];
-            for (int e = threadIdx.x; e < tn * wn; e += MQE_NT) {
-                const int a = e % tn, bq = e / tn;
-                const size_t o = a + (size_t)2 * tn * bq;
-                const T g00 = G[o], g01 = G[o + (size_t)wn * 2 * tn], g10 = G[o + tn],
-                        g11 = G[o + tn + (size_t)wn * 2 * tn];
-                Ln[a + (size_t)tn * bq] = u00 * g00 + u01 * g01 + u10 * g10 + u11 * g11;
+            const T v00 = (T)(alpha * Vx[0]), v10 = (T)(alpha * Vx[1]), v01 = (T)(alpha * Vx[2]),
+                    v11 = (T)(alpha * Vx[3]);
+            for (int e = threadIdx.x; e < bi; e += MQE_NT) {
+                const T a0 = pi[e], a1 = pi[e + bi];
+                pi[e] = u00 * a0 + u01 * a1;
+                pi[e + bi] = u10 * a0 + u11 * a1;
+            }
+            for (int e = threadIdx.x; e < bj; e += MQE_NT) {
+                const T a0 = pj[e], a1 = pj[e + bj];
+                pj[e] = v00 * a0 + v01 * a1;
+                pj[e + bj] = v10 * a0 + v11 * a1;
+            }
+            if (nn_move) {
+                const int tn = tv[i + 1], wn = wv[i + 1];
+                const T *G = C.ts + C.sl.G;
+                T *Ln = C.Lk(i + 1);
+                for (int e = threadIdx.x; e < tn * wn; e += MQE_NT) {
+                    const int a = e % tn, bq = e / tn;
+                    const size_t o = a + (size_t)2 * tn * bq;
+                    const T g00 = G[o], g01 = G[o + (size_t)wn * 2 * tn], g10 = G[o + tn],
+                            g11 = G[o + tn + (size_t)wn * 2 * tn];
+                    Ln[a + (size_t)tn * bq] = u00 * g00 + u01 * g01 + u10 * g10 + u11 * g11;
+                }
             }
             __syncthreads();
-            p = i + 1;
-        } else {
-            p = i;  // site i changed: L_{i+1} is rebuilt when needed
         }
+        p = nn_move ? i + 1 : i;  // else site i changed: L_{i+1} is rebuilt when needed
         rv = max(rv, j + 1);  // site j changed
     }
     if (S.h != 0.0) {
