@@ -334,7 +334,8 @@ __device__ __forceinline__ void compute_V(const Ctx<T, CH> &C, int j) {
 // Carry four channels (t_k x w_k, ld t_k, contiguous at src) through site k
 // into dst (t_{k+1} x w_{k+1}): dst_ch = sum_s T_k[s]^T src_ch phi_k[s].
 template <typename T, bool CH>
-__device__ __forceinline__ void carry_channels(const Ctx<T, CH> &C, int k, const T *src, T *dst) {
+// nch: the number of channels (4, or 3 for the operator channels of a bond energy).
+__device__ __forceinline__ void carry_channels(const Ctx<T, CH> &C, int k, const T *src, T *dst, int nch = 4) {
     const int tk = C.tv[k], tn = C.tv[k + 1], wk = C.wv[k], wn = C.wv[k + 1];
     const T *Tk = C.tsite(k);
     if constexpr (sizeof(T) == sizeof(float)) {
@@ -351,19 +352,20 @@ __device__ __forceinline__ void carry_channels(const Ctx<T, CH> &C, int k, const
                 return o;
             };
             if (wn < 32)
-                chain2_16<false, true, 2, 1>(0, 4, tk, wn, wk, tn, tk, wk, tk, tn, false, unit);
+                chain2_16<false, true, 2, 1>(0, nch, tk, wn, wk, tn, tk, wk, tk, tn, false, unit);
             else
-                chain2<false, true, 2, 1>(0, 4, tk, wn, wk, tn, tk, wk, tk, tn, false, unit);
+                chain2<false, true, 2, 1>(0, nch, tk, wn, wk, tn, tk, wk, tk, tn, false, unit);
             __syncthreads();
             return;
         }
     }
     T *Y = C.ts + C.sl.Y4;  // Y_ch = src_ch phi_k (tk x 2 wn), the four channels one after the other
     const long sy = (long)tk * 2 * wn;
-    gemm_batch<false, false, T>(4, tk, 2 * wn, wk, src, tk, (long)tk * wk, C.site(k), wk, 0, Y, tk, sy, false, C.tile);
+    gemm_batch<false, false, T>(nch, tk, 2 * wn, wk, src, tk, (long)tk * wk, C.site(k), wk, 0, Y, tk, sy, false,
+                                C.tile);
     __syncthreads();
     // dst_ch = T_k[0]^T Y_ch[0] + T_k[1]^T Y_ch[1], one accumulation chain per output tile
-    gemm2<true, false, T>(tn, wn, tk, Tk, Tk + (size_t)tk * tn, tk, Y, Y + (size_t)tk * wn, tk, dst, tn, C.tile, 4, 0,
+    gemm2<true, false, T>(tn, wn, tk, Tk, Tk + (size_t)tk * tn, tk, Y, Y + (size_t)tk * wn, tk, dst, tn, C.tile, nch, 0,
                           sy, (long)tn * wn);
     __syncthreads();
 }
@@ -407,6 +409,27 @@ __device__ __forceinline__ void pair_tensor(const Ctx<T, CH> &C, int tb, int wb,
         for (int q = 0; q < 16; ++q) M[q] = acc[q];
     }
     __syncthreads();
+}
+
+// Bond energy <trial|S_i . S_j|phi> from the three operator channels of site i
+// carried to bond j (c[0] = S^z, c[1] = S^+, c[2] = S^- applied at site i; t_j x
+// w_j, ld t_j) and the blocks V(t', t) of site j:
+// S_i . S_j = S^z S^z + (S^+ S^- + S^- S^+) / 2, with S^z = diag(-1/2, 1/2),
+// S^+ = |1><0|, S^- = |0><1| in the basis ordered by m (OpSz.cpp, OpSx.cpp,
+// OpISy.cpp).  Rout (or nullptr): also writes R_j = V(0,0) + V(1,1) there.
+template <typename T, bool CH>
+__device__ __forceinline__ double pair_energy3(const Ctx<T, CH> &C, int tb, int wb, const T *c, T *Rout = nullptr) {
+    const T *V = C.ts + C.sl.V;
+    const size_t bs = (size_t)tb * wb;
+    const T *V00 = V, *V01 = V + bs, *V10 = V + 2 * bs, *V11 = V + 3 * bs;  // V(t', t) at t' 2 bs + t bs
+    double acc = 0.0;
+    for (int e = threadIdx.x; e < (int)bs; e += MQE_NT) {
+        const T v00 = V00[e], v11 = V11[e];
+        acc += (double)c[e] * (0.5 * ((double)v11 - (double)v00)) +
+               0.5 * ((double)c[bs + e] * (double)V01[e] + (double)c[2 * bs + e] * (double)V10[e]);
+        if (Rout) Rout[e] = v00 + v11;
+    }
+    return block_sum(acc, C.red);
 }
 
 // Mixed tensor for the pair (i, j) from L_i, sites i..j and R_{j+1}.
@@ -766,15 +789,31 @@ __device__ __forceinline__ double energy_sweep(const Ctx<T, CH> &C, double *O_ou
                 }
                 gld = 2 * tn;
             } else {
+                // long range: only S_i . S_k is needed, so the four channels G(s', s)
+                // are combined into the three operator channels S^z, S^+, S^- of
+                // site i (ch = 2 s' + s: S^z = (G(1,1) - G(0,0)) / 2, S^+ = G(1,0),
+                // S^- = G(0,1)) and three are carried instead of four
                 T *A = C.ts + C.sl.GcA, *B = C.ts + C.sl.GcB;
                 compute_G_channels(C, i, A);
                 if constexpr (CH) __syncthreads();
+                {
+                    const int nb = tn * wn;
+                    for (int e = threadIdx.x; e < nb; e += MQE_NT) {
+                        const T g00 = A[e], g01 = A[nb + e], g10 = A[2 * nb + e], g11 = A[3 * nb + e];
+                        A[e] = (T)0.5 * (g11 - g00);
+                        A[nb + e] = g10;
+                        A[2 * nb + e] = g01;
+                    }
+                    __syncthreads();
+                }
                 for (int kk = i + 1; kk < k; ++kk) {
-                    carry_channels(C, kk, A, B);
+                    carry_channels(C, kk, A, B, 3);
                     T *t = A; A = B; B = t;
                 }
-                for (int ch = 0; ch < 4; ++ch) g[ch] = A + (size_t)ch * C.tv[k] * C.wv[k];
-                gld = C.tv[k];
+                const double e = pair_energy3(C, C.tv[k], C.wv[k], A, r_pending ? C.Rk(k) : nullptr);
+                r_pending = false;
+                etot += S.Jc[c] * e;
+                continue;
             }
             pair_tensor(C, C.tv[k], C.wv[k], g, gld, M, r_pending ? C.Rk(k) : nullptr);
             r_pending = false;
