@@ -475,29 +475,34 @@ __device__ __forceinline__ void pair_for_coupling(const Ctx<T, CH> &C, int i, in
 
 // L_{k+1} = sum_s T_k[s]^T L_k phi_k[s]
 template <typename T, bool CH>
-__device__ __forceinline__ void advance_L(const Ctx<T, CH> &C, int k) {
-    const int tk = C.tv[k], tn = C.tv[k + 1], wk = C.wv[k], wn = C.wv[k + 1];
-    if constexpr (sizeof(T) == sizeof(float)) {
-        if constexpr (CH) {
-            const float *ph = (const float *)C.site(k), *Tk = (const float *)C.tsite(k);
-            auto unit = [&](int) {
-                ChainOps o;
-                o.A = (const float *)C.Lk(k);
-                o.B[0] = ph;
-                o.B[1] = ph + (size_t)wk * wn;
-                o.C[0] = Tk;
-                o.C[1] = Tk + (size_t)tk * tn;
-                o.D[0] = o.D[1] = (float *)C.Lk(k + 1);
-                return o;
-            };
-            if (wn < 32)
-                chain2_16<false, true, 2, 1>(0, 1, tk, wn, wk, tn, tk, wk, tk, tn, true, unit);
-            else
-                chain2<false, true, 2, 1>(0, 1, tk, wn, wk, tn, tk, wk, tk, tn, true, unit);
-            __syncthreads();
-            return;
-        }
+__device__ __forceinline__ int issue_L(const Ctx<T, CH> &C, int k, int wave0) {
+    if constexpr (CH) {
+        const int tk = C.tv[k], tn = C.tv[k + 1], wk = C.wv[k], wn = C.wv[k + 1];
+        const float *ph = (const float *)C.site(k), *Tk = (const float *)C.tsite(k);
+        auto unit = [&](int) {
+            ChainOps o;
+            o.A = (const float *)C.Lk(k);
+            o.B[0] = ph;
+            o.B[1] = ph + (size_t)wk * wn;
+            o.C[0] = Tk;
+            o.C[1] = Tk + (size_t)tk * tn;
+            o.D[0] = o.D[1] = (float *)C.Lk(k + 1);
+            return o;
+        };
+        if (wn < 32) return chain2_16<false, true, 2, 1>(wave0, 1, tk, wn, wk, tn, tk, wk, tk, tn, true, unit);
+        return chain2<false, true, 2, 1>(wave0, 1, tk, wn, wk, tn, tk, wk, tk, tn, true, unit);
     }
+    return wave0;
+}
+
+template <typename T, bool CH>
+__device__ __forceinline__ void advance_L(const Ctx<T, CH> &C, int k) {
+    if constexpr (CH) {
+        issue_L(C, k, 0);
+        __syncthreads();
+        return;
+    }
+    const int tk = C.tv[k], tn = C.tv[k + 1], wk = C.wv[k], wn = C.wv[k + 1];
     T *Y = C.ts + C.sl.Y;
     gemm<false, false, T>(tk, 2 * wn, wk, C.Lk(k), tk, C.site(k), wk, Y, tk, false, C.tile);
     __syncthreads();
@@ -509,28 +514,32 @@ __device__ __forceinline__ void advance_L(const Ctx<T, CH> &C, int k) {
 
 // R_k = sum_t T_k[t] R_{k+1} phi_k[t]^T (the diagonal blocks of V at bond k)
 template <typename T, bool CH>
-__device__ __forceinline__ void rebuild_R(const Ctx<T, CH> &C, int k) {
-    if constexpr (sizeof(T) == sizeof(float)) {
+__device__ __forceinline__ int issue_R(const Ctx<T, CH> &C, int k, int wave0) {
+    if constexpr (CH) {
         const int tk = C.tv[k], tn = C.tv[k + 1], wk = C.wv[k], wn = C.wv[k + 1];
-        if constexpr (CH) {
-            const float *ph = (const float *)C.site(k), *TTk = C.tsiteT(k);
-            auto unit = [&](int) {
-                ChainOps o;
-                o.A = (const float *)C.Rk(k + 1);
-                o.B[0] = ph;
-                o.B[1] = ph + (size_t)wk * wn;
-                o.C[0] = TTk;
-                o.C[1] = TTk + (size_t)tk * tn;
-                o.D[0] = o.D[1] = (float *)C.Rk(k);
-                return o;
-            };
-            if (wk < 32)
-                chain2_16<true, true, 2, 1>(0, 1, tn, wk, wn, tk, tn, wk, tn, tk, true, unit);
-            else
-                chain2<true, true, 2, 1>(0, 1, tn, wk, wn, tk, tn, wk, tn, tk, true, unit);
-            __syncthreads();
-            return;
-        }
+        const float *ph = (const float *)C.site(k), *TTk = C.tsiteT(k);
+        auto unit = [&](int) {
+            ChainOps o;
+            o.A = (const float *)C.Rk(k + 1);
+            o.B[0] = ph;
+            o.B[1] = ph + (size_t)wk * wn;
+            o.C[0] = TTk;
+            o.C[1] = TTk + (size_t)tk * tn;
+            o.D[0] = o.D[1] = (float *)C.Rk(k);
+            return o;
+        };
+        if (wk < 32) return chain2_16<true, true, 2, 1>(wave0, 1, tn, wk, wn, tk, tn, wk, tn, tk, true, unit);
+        return chain2<true, true, 2, 1>(wave0, 1, tn, wk, wn, tk, tn, wk, tn, tk, true, unit);
+    }
+    return wave0;
+}
+
+template <typename T, bool CH>
+__device__ __forceinline__ void rebuild_R(const Ctx<T, CH> &C, int k) {
+    if constexpr (CH) {
+        issue_R(C, k, 0);
+        __syncthreads();
+        return;
     }
     compute_V(C, k);
     const int tk = C.tv[k], wk = C.wv[k];
@@ -1023,11 +1032,22 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker_kernel(Sys S, Pool P, const 
     int p = 0, rv = 1;
     for (int c = 0; c < S.nC; ++c) {
         const int i = S.first[c], j = S.second[c];
-        while (p < i) {
-            advance_L(C, p);
-            ++p;
+        // the left environments up to L_i and the right ones down to R_{j+1}: two
+        // independent chains (CH: one step of each per barrier interval)
+        if constexpr (CH) {
+            while (p < i || rv > j + 1) {
+                int wq = 0;
+                if (p < i) wq = issue_L(C, p++, wq);
+                if (rv > j + 1) issue_R(C, --rv, wq);
+                __syncthreads();
+            }
+        } else {
+            while (p < i) {
+                advance_L(C, p);
+                ++p;
+            }
+            while (rv > j + 1) rebuild_R(C, --rv);
         }
-        while (rv > j + 1) rebuild_R(C, --rv);
         pair_for_coupling(C, i, j, M);
         const double *tS = S.tS + 4 * c, *tU = S.tU + 16 * c, *tVT = S.tVT + 16 * c;
         const double u = P.u ? P.u[(size_t)b * S.nC + c] : philox_uniform(P.seed, (uint32_t)b, (uint32_t)c, (uint32_t)ctl.step, 0u);
