@@ -1412,17 +1412,24 @@ __global__ void advance_ctl_kernel(Ctl *ctl) {
 __global__ __launch_bounds__(RED_NT) void launch_list_kernel(const int *n_ptr, const int *order, const int *wvd,
                                                              const int *since, int L, const Ctl *ctl, int *key,
                                                              int *list) {
-    __shared__ int shi[RED_NT / 64];
-    __shared__ int cnt[16], base[16];
-    __shared__ int carry;
+    __shared__ int cnt[16], base[16], carry[16];
+    __shared__ int wcnt[RED_NT / 64][16];
     const int n = *n_ptr;
     const Ctl c = *ctl;
-    if (threadIdx.x < 16) cnt[threadIdx.x] = 0;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (threadIdx.x < 16) {
+        cnt[threadIdx.x] = 0;
+        carry[threadIdx.x] = 0;
+    }
     __syncthreads();
     for (int b = threadIdx.x; b < n; b += RED_NT) {
         const int slot = order[b];
+        const int *wv = wvd + (size_t)slot * (L + 1);
         int mb = 1;
-        for (int k = 0; k <= L; ++k) mb = max(mb, wvd[(size_t)slot * (L + 1) + k]);
+        // eight independent loads in flight at a time (one at a time, the loop
+        // waited L + 1 memory latencies per walker)
+#pragma unroll 8
+        for (int k = 0; k <= L; ++k) mb = max(mb, wv[k]);
         const int kk = (canon_walker(c, slot, since[slot]) ? 8 : 0) + bond_bucket(mb);
         key[b] = kk;
         atomicAdd(&cnt[kk], 1);
@@ -1432,21 +1439,34 @@ __global__ __launch_bounds__(RED_NT) void launch_list_kernel(const int *n_ptr, c
         int run = 0;
         for (int kk = 15; kk >= 0; --kk) { base[kk] = run; run += cnt[kk]; }
     }
-    __syncthreads();
-    for (int kk = 15; kk >= 0; --kk) {
-        if (cnt[kk] == 0) continue;  // block-uniform
-        if (threadIdx.x == 0) carry = base[kk];
-        __syncthreads();
-        for (int b0 = 0; b0 < n; b0 += RED_NT) {
-            const int b = b0 + threadIdx.x;
-            const int f = (b < n && key[b] == kk) ? 1 : 0;
-            int ex;
-            const int tot = rscan_excl(f, &ex, shi);
-            if (f) list[carry + ex] = b;
-            __syncthreads();
-            if (threadIdx.x == 0) carry += tot;
-            __syncthreads();
+    int present = 0;  // classes that occur (block-uniform)
+    for (int kk = 0; kk < 16; ++kk) present |= (cnt[kk] > 0 ? 1 : 0) << kk;
+    // stable placement in one pass: a position's rank among the positions of its
+    // class before it is its rank inside its wave (ballot) plus the counts of the
+    // earlier waves of the chunk and of the earlier chunks
+    for (int b0 = 0; b0 < n; b0 += RED_NT) {
+        const int b = b0 + threadIdx.x;
+        const int kk = b < n ? key[b] : -1;
+        int rank = 0;
+        for (int q = 0; q < 16; ++q) {
+            if (!((present >> q) & 1)) continue;
+            const unsigned long long m = __ballot(kk == q);
+            if (kk == q) rank = __popcll(m & ((1ull << lane) - 1ull));
+            if (lane == 0) wcnt[wid][q] = __popcll(m);
         }
+        __syncthreads();
+        if (kk >= 0) {
+            int off = base[kk] + carry[kk] + rank;
+            for (int w = 0; w < wid; ++w) off += wcnt[w][kk];
+            list[off] = b;
+        }
+        __syncthreads();
+        if (threadIdx.x < 16 && ((present >> threadIdx.x) & 1)) {
+            int t = 0;
+            for (int w = 0; w < RED_NT / 64; ++w) t += wcnt[w][threadIdx.x];
+            carry[threadIdx.x] += t;
+        }
+        __syncthreads();
     }
 }
 
