@@ -363,10 +363,21 @@ __device__ __forceinline__ void scale_blocks(float *b, int nb, double f) {
 // phi[s] <- sum_s0 O[s + 2 s0] phi[s0] on a 2-block LDS tensor
 __device__ __forceinline__ void apply2(float *b, const double *O) {
     const float o00 = (float)O[0], o10 = (float)O[1], o01 = (float)O[2], o11 = (float)O[3];
-    for (int e = threadIdx.x; e < BLK; e += MQE_NT) {
-        const float a0 = b[e], a1 = b[e + BLK];
-        b[e] = o00 * a0 + o01 * a1;
-        b[e + BLK] = o10 * a0 + o11 * a1;
+    // the 32 x 32 entries of each block (the padding row is never read); all
+    // loads of the thread first, then the stores
+    constexpr int NE = GB / MQE_NT;
+    float a[NE][2];
+#pragma unroll
+    for (int it = 0; it < NE; ++it) {
+        const int e = threadIdx.x + it * MQE_NT, o = (e & 31) + LD * (e >> 5);
+        a[it][0] = b[o];
+        a[it][1] = b[o + BLK];
+    }
+#pragma unroll
+    for (int it = 0; it < NE; ++it) {
+        const int e = threadIdx.x + it * MQE_NT, o = (e & 31) + LD * (e >> 5);
+        b[o] = o00 * a[it][0] + o01 * a[it][1];
+        b[o + BLK] = o10 * a[it][0] + o11 * a[it][1];
     }
 }
 
@@ -803,23 +814,21 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
             pv[t] = v;
             loc += v;
         }
-        const double pdfnorm = wave_sum(loc);
-        const double inv = 1.0 / pdfnorm;
-        double ln = 0.0;
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            pv[t] *= inv;
-            ln += pv[t];
-        }
+        // one scan of the unnormalised PDF gives its sum (lane 63) and the CDF;
+        // cum_g <= u is tested as cum_g * norm <= u * norm, and the count of
+        // grid points that pass is a sum of four ballots
+        const double incl = wave_inclusive_scan(loc);
+        const double pdfnorm = readlane_d(incl, 63);
         const double u = P.u ? P.u[(size_t)b * S.nC + c]
                              : philox_uniform(P.seed, (uint32_t)b, (uint32_t)c, (uint32_t)ctl.step, 0u);
-        double run = wave_exclusive_scan(ln), cnt = 0.0;
+        const double ut = u * pdfnorm;
+        double run = dpp_d<0x138>(incl);  // exclusive prefix (wave_shr:1)
+        int gs = 0;
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             run += pv[t];
-            if (4 * lane + t < Ng - 1 && run <= u) cnt += 1.0;
+            gs += __popcll(__ballot(4 * lane + t < Ng - 1 && run <= ut));
         }
-        int gs = (int)wave_sum(cnt);
         if constexpr (LEARN) {
             // learned proposal q from the predicted labels; weight pdf(x)/q(x)
             double ph[16];
@@ -852,13 +861,13 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
                 qv[t] = 4 * lane + t < Ng ? a1 * qv[t] + a0 : 0.0;
                 ql += qv[t];
             }
-            double qrun = wave_exclusive_scan(ql), qcnt = 0.0;
+            double qrun = wave_exclusive_scan(ql);
+            int gl = 0;
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
                 qrun += qv[t];
-                if (4 * lane + t < Ng - 1 && qrun <= u) qcnt += 1.0;
+                gl += __popcll(__ballot(4 * lane + t < Ng - 1 && qrun <= u));
             }
-            const int gl = (int)wave_sum(qcnt);
             const double *xl = grid_s + 4 * gl;
             const double vl = fmax(quad(cb, xl), 0.0) * pref;
             const double qgl = a1 * fmax(quad(ph, xl), 0.0) + a0;
@@ -910,14 +919,26 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
             float *Ln = C.blk(B_L), *lg = C.Lg + (size_t)(i + 1) * GB;
             float *pg0 = C.ph(i, 0), *pg1 = C.ph(i, 1);
             const float u0 = (float)Ux[0], u1 = (float)Ux[1], u2 = (float)Ux[2], u3 = (float)Ux[3];
-            for (int e = threadIdx.x; e < GB; e += MQE_NT) {
-                const int o = (e & 31) + LD * (e >> 5);
-                const float v = u0 * G[o] + u2 * G[BLK + o] + u1 * G[2 * BLK + o] + u3 * G[3 * BLK + o];
+            // every LDS operand of the thread's four elements is loaded before
+            // the first store (a store to B_L could otherwise not pass the loads)
+            constexpr int NE = GB / MQE_NT;
+            float gv[NE][4], av[NE][2];
+#pragma unroll
+            for (int it = 0; it < NE; ++it) {
+                const int e = threadIdx.x + it * MQE_NT, o = (e & 31) + LD * (e >> 5);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) gv[it][q] = G[q * BLK + o];
+                av[it][0] = cen[o];
+                av[it][1] = cen[BLK + o];
+            }
+#pragma unroll
+            for (int it = 0; it < NE; ++it) {
+                const int e = threadIdx.x + it * MQE_NT, o = (e & 31) + LD * (e >> 5);
+                const float v = u0 * gv[it][0] + u2 * gv[it][1] + u1 * gv[it][2] + u3 * gv[it][3];
                 Ln[o] = v;
                 lg[e] = v;
-                const float a0 = cen[o], a1 = cen[BLK + o];
-                pg0[e] = u0 * a0 + u2 * a1;
-                pg1[e] = u1 * a0 + u3 * a1;
+                pg0[e] = u0 * av[it][0] + u2 * av[it][1];
+                pg1[e] = u1 * av[it][0] + u3 * av[it][1];
             }
             apply2(nxt, Vx);
             if (c + 1 < S.nC) {
