@@ -699,6 +699,11 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
         float *cen = C.blk(pc), *nxt = C.blk(pj);
         float plab = 0.f;  // learned proposal: predicted label lane of this coupling (lanes < 16)
         if constexpr (LEARN) plab = lane < 16 ? P.pred[((size_t)b * S.nC + c) * 16 + lane] : 0.f;
+        // the coupling's uniform, drawn in stage AB so that the counter-based
+        // generator (or the load) runs between the matrix-core products instead
+        // of on the sampling stage's critical path (pinned there below)
+        double u = P.u ? P.u[(size_t)b * S.nC + c]
+                       : philox_uniform(P.seed, (uint32_t)b, (uint32_t)c, (uint32_t)ctl.step, 0u);
         // trial operands of stage AB: T_i[s']^T (waves 0,1) or T_j[t'] (waves 2,3), s', t' = 0, 1
         frag16 f0, f1;
         if (w < 2) {
@@ -731,6 +736,7 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
             store(r0, base + (w & 1) * BLK, LD);
             store(r1, base + (2 + (w & 1)) * BLK, LD);
         }
+        asm volatile("" : "+v"(u));
         __syncthreads();
         // stage C: the 16 two-site mixed matrix elements M[(s'+2t')4 + (s+2t)] = <G(s',s), V(t',t)>
         //   waves 0,1: <G(0,s), V(t',t)> for all t', t;  waves 2,3: <V(t',t), G(1,s)> for all t', s
@@ -760,11 +766,11 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
         if (lane < 16) {
             // combo (a, b) = sqrt(S_a S_b) sum U_a(s',s) VT_b(t',t) M[(s'+2t')4 + (s+2t)], operands loaded first
             const int a = lane & 3, bb = lane >> 2;
-            double u[4], v[4], m[16];
+            double ua[4], vb[4], m[16];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                u[q] = tU[(q & 1) + 2 * ((q >> 1) + 2 * a)];    // u[s' + 2s] = U_a(s', s)
-                v[q] = tVT[bb + 4 * ((q & 1) + 2 * (q >> 1))];  // v[t' + 2t] = VT_b(t', t)
+                ua[q] = tU[(q & 1) + 2 * ((q >> 1) + 2 * a)];    // ua[s' + 2s] = U_a(s', s)
+                vb[q] = tVT[bb + 4 * ((q & 1) + 2 * (q >> 1))];  // vb[t' + 2t] = VT_b(t', t)
             }
 #pragma unroll
             for (int q = 0; q < 16; ++q) m[q] = M[q];
@@ -777,8 +783,8 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
 #pragma unroll
                     for (int tp = 0; tp < 2; ++tp)
 #pragma unroll
-                        for (int t = 0; t < 2; ++t) mv += v[tp + 2 * t] * m[(xsp + 2 * tp) * 4 + (xss + 2 * t)];
-                    sacc += u[xsp + 2 * xss] * mv;
+                        for (int t = 0; t < 2; ++t) mv += vb[tp + 2 * t] * m[(xsp + 2 * tp) * 4 + (xss + 2 * t)];
+                    sacc += ua[xsp + 2 * xss] * mv;
                 }
             cq = sS[a] * sS[bb] * sacc;
         }
@@ -794,21 +800,35 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
                 if (base + (long)n * S.nC <= P.maxsamp) P.lab[row * 16 + lane] = lv;
             }
         }
+        // f(x) = sum_{l,r} comb[l + 4r] x_l x_r through the symmetric part of
+        // the 4 x 4 combo matrix: hs = (C_ll, C_lr + C_rl for r > l), row by row
+        double hs[10];
+        {
+            int k = 0;
+#pragma unroll
+            for (int l = 0; l < 4; ++l)
+#pragma unroll
+                for (int r = l; r < 4; ++r) hs[k++] = r == l ? cb[5 * l] : cb[l + 4 * r] + cb[r + 4 * l];
+        }
+        auto quad_s = [&](const double *x) {
+            double f = 0.0;
+            int k = 0;
+#pragma unroll
+            for (int l = 0; l < 4; ++l) {
+                double t = 0.0;
+#pragma unroll
+                for (int r = l; r < 4; ++r) t += hs[k++] * x[r];
+                f += x[l] * t;
+            }
+            return f;
+        };
         double pv[4], loc = 0.0;
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const int gq = 4 * lane + t;
             double v = 0.0;
             if (gq < Ng) {
-                const double *x = grid_s + 4 * gq;
-                double f = 0.0;  // sum_l x_l (sum_r comb[l + 4r] x_r)
-#pragma unroll
-                for (int l = 0; l < 4; ++l) {
-                    double t = 0.0;
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) t += cb[l + 4 * r] * x[r];
-                    f += x[l] * t;
-                }
+                const double f = quad_s(grid_s + 4 * gq);
                 v = (f > 0.0 ? f : 0.0) * pref;
             }
             pv[t] = v;
@@ -819,8 +839,6 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
         // grid points that pass is a sum of four ballots
         const double incl = wave_inclusive_scan(loc);
         const double pdfnorm = readlane_d(incl, 63);
-        const double u = P.u ? P.u[(size_t)b * S.nC + c]
-                             : philox_uniform(P.seed, (uint32_t)b, (uint32_t)c, (uint32_t)ctl.step, 0u);
         const double ut = u * pdfnorm;
         double run = dpp_d<0x138>(incl);  // exclusive prefix (wave_shr:1)
         int gs = 0;
@@ -885,14 +903,7 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker32_kernel(Sys S, Pool P, cons
             if (!(pdfnorm > 0.0) || !isfinite(pdfnorm)) atomicOr(P.status, 1);
         }
         const double *xs = grid_s + 4 * gs;
-        double fstar = 0.0;  // <trial|B(x*) phi> / 4
-#pragma unroll
-        for (int l = 0; l < 4; ++l) {
-            double t = 0.0;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) t += cb[l + 4 * r] * xs[r];
-            fstar += xs[l] * t;
-        }
+        const double fstar = quad_s(xs);  // <trial|B(x*) phi> / 4
         double Ux[4] = {0.0, 0.0, 0.0, 0.0}, Vx[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
