@@ -488,12 +488,25 @@ __device__ double energy_sweep32(const W32 &C, int pc, double *On, double *Ou, b
                 }
                 float *R = C.blk(B_R), *Mk = C.blk(B_LM);
                 float *rg = C.Rg + (size_t)k * GB;
-                for (int e = threadIdx.x; e < GB; e += MQE_NT) {
-                    const int o = (e & 31) + LD * (e >> 5);
-                    const float rv = V[o] + V[3 * BLK + o];
+                // all LDS loads of the thread's four elements before the first
+                // store (stores to B_R / B_LM could otherwise not pass them)
+                constexpr int NE = GB / MQE_NT;
+                float vv[NE][2], mv[NE][2];
+#pragma unroll
+                for (int it = 0; it < NE; ++it) {
+                    const int e = threadIdx.x + it * MQE_NT, o = (e & 31) + LD * (e >> 5);
+                    vv[it][0] = V[o];
+                    vv[it][1] = V[3 * BLK + o];
+                    mv[it][0] = Mt[o];
+                    mv[it][1] = Mt[BLK + o];
+                }
+#pragma unroll
+                for (int it = 0; it < NE; ++it) {
+                    const int e = threadIdx.x + it * MQE_NT, o = (e & 31) + LD * (e >> 5);
+                    const float rv = vv[it][0] + vv[it][1];
                     R[o] = rv;
                     rg[e] = rv;
-                    Mk[o] = Mt[o] + Mt[BLK + o];
+                    Mk[o] = mv[it][0] + mv[it][1];
                 }
                 if (k >= 2) {  // phi_{k-2} into the buffer of phi_k (not read in this stage), L_{k-2}
                     float *const d[3] = {cen, cen + BLK, C.blk(B_L)};
