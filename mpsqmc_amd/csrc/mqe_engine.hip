@@ -613,17 +613,37 @@ __device__ __forceinline__ void scale_site(const Ctx<T, CH> &C, int k, double f)
     __syncthreads();
 }
 
+// (x[e], x[e + n]) <- (c00 x[e] + c01 x[e + n], c10 x[e] + c11 x[e + n]) for
+// e < n: a 2 x 2 operator on the physical index of a site stored as two blocks
+// of n.  Four elements of a thread are in flight at a time, their loads issued
+// before their stores (one element at a time, every load waited for the store
+// before it: the compiler cannot tell that they do not overlap).
+template <typename T>
+__device__ __forceinline__ void mix_pair(T *x, int n, T c00, T c01, T c10, T c11) {
+    for (int e0 = threadIdx.x; e0 < n; e0 += 4 * MQE_NT) {
+        T a[4], b[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = e0 + q * MQE_NT;
+            a[q] = e < n ? x[e] : T(0);
+            b[q] = e < n ? x[e + n] : T(0);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = e0 + q * MQE_NT;
+            if (e < n) {
+                x[e] = c00 * a[q] + c01 * b[q];
+                x[e + n] = c10 * a[q] + c11 * b[q];
+            }
+        }
+    }
+}
+
 // phi_k[s] <- sum_s0 O[s + 2 s0] phi_k[s0]
 template <typename T, bool CH>
 __device__ __forceinline__ void apply_op(const Ctx<T, CH> &C, int k, const double *O) {
-    T *ph = C.site(k);
-    const int blk = C.wv[k] * C.wv[k + 1];
     const T o00 = (T)O[0], o10 = (T)O[1], o01 = (T)O[2], o11 = (T)O[3];
-    for (int e = threadIdx.x; e < blk; e += MQE_NT) {
-        const T a0 = ph[e], a1 = ph[e + blk];
-        ph[e] = o00 * a0 + o01 * a1;
-        ph[e + blk] = o10 * a0 + o11 * a1;
-    }
+    mix_pair(C.site(k), C.wv[k] * C.wv[k + 1], o00, o01, o10, o11);
     __syncthreads();
 }
 
@@ -1122,16 +1142,8 @@ __global__ __launch_bounds__(MQE_NT, 2) void walker_kernel(Sys S, Pool P, const 
             const T u00 = (T)Ux[0], u10 = (T)Ux[1], u01 = (T)Ux[2], u11 = (T)Ux[3];
             const T v00 = (T)(alpha * Vx[0]), v10 = (T)(alpha * Vx[1]), v01 = (T)(alpha * Vx[2]),
                     v11 = (T)(alpha * Vx[3]);
-            for (int e = threadIdx.x; e < bi; e += MQE_NT) {
-                const T a0 = pi[e], a1 = pi[e + bi];
-                pi[e] = u00 * a0 + u01 * a1;
-                pi[e + bi] = u10 * a0 + u11 * a1;
-            }
-            for (int e = threadIdx.x; e < bj; e += MQE_NT) {
-                const T a0 = pj[e], a1 = pj[e + bj];
-                pj[e] = v00 * a0 + v01 * a1;
-                pj[e + bj] = v10 * a0 + v11 * a1;
-            }
+            mix_pair(pi, bi, u00, u01, u10, u11);
+            mix_pair(pj, bj, v00, v01, v10, v11);
             if (nn_move) {
                 const int tn = tv[i + 1], wn = wv[i + 1];
                 const T *G = C.ts + C.sl.G;
