@@ -381,27 +381,39 @@ __device__ __forceinline__ void pair_tensor(const Ctx<T, CH> &C, int tb, int wb,
 #pragma unroll
     for (int q = 0; q < 16; ++q) acc[q] = 0.0;
     const int nel = tb * wb;
-    for (int e = threadIdx.x; e < nel; e += MQE_NT) {
-        const int a = e % tb, b = e / tb;
-        double gv[4], vv[4];
-        T vt[4];
+    // two elements of a thread per pass, the loads of both before the store to
+    // Rout (which the compiler cannot separate from them); the second index is
+    // clamped into the block, its element skipped below (same order of sums)
+    for (int e0 = threadIdx.x; e0 < nel; e0 += 2 * MQE_NT) {
+        T gt[2][4], vt[2][4];
 #pragma unroll
-        for (int ch = 0; ch < 4; ++ch) gv[ch] = (double)g[ch][a + (size_t)gld * b];
+        for (int u = 0; u < 2; ++u) {
+            const int e = min(e0 + u * MQE_NT, nel - 1);
+            const int a = e % tb, b = e / tb;
 #pragma unroll
-        for (int ch = 0; ch < 4; ++ch) {
-            const int tp = ch >> 1, t = ch & 1;
-            vt[ch] = V[(size_t)tp * tb * 2 * wb + (size_t)t * tb * wb + a + (size_t)tb * b];
-            vv[ch] = (double)vt[ch];
+            for (int ch = 0; ch < 4; ++ch) {
+                const int tp = ch >> 1, t = ch & 1;
+                gt[u][ch] = g[ch][a + (size_t)gld * b];
+                vt[u][ch] = V[(size_t)tp * tb * 2 * wb + (size_t)t * tb * wb + a + (size_t)tb * b];
+            }
         }
-        if (Rout) Rout[e] = vt[0] + vt[3];
 #pragma unroll
-        for (int sp = 0; sp < 2; ++sp)
+        for (int u = 0; u < 2; ++u) {
+            const int e = e0 + u * MQE_NT;
+            if (e < nel) {
+                if (Rout) Rout[e] = vt[u][0] + vt[u][3];
 #pragma unroll
-            for (int s = 0; s < 2; ++s)
+                for (int sp = 0; sp < 2; ++sp)
 #pragma unroll
-                for (int tp = 0; tp < 2; ++tp)
+                    for (int s = 0; s < 2; ++s)
 #pragma unroll
-                    for (int t = 0; t < 2; ++t) acc[(sp + 2 * tp) * 4 + (s + 2 * t)] += gv[2 * sp + s] * vv[2 * tp + t];
+                        for (int tp = 0; tp < 2; ++tp)
+#pragma unroll
+                            for (int t = 0; t < 2; ++t)
+                                acc[(sp + 2 * tp) * 4 + (s + 2 * t)] +=
+                                    (double)gt[u][2 * sp + s] * (double)vt[u][2 * tp + t];
+            }
+        }
     }
     block_sum_n<16>(acc, C.red);
     if (threadIdx.x == 0) {
@@ -423,11 +435,31 @@ __device__ __forceinline__ double pair_energy3(const Ctx<T, CH> &C, int tb, int 
     const size_t bs = (size_t)tb * wb;
     const T *V00 = V, *V01 = V + bs, *V10 = V + 2 * bs, *V11 = V + 3 * bs;  // V(t', t) at t' 2 bs + t bs
     double acc = 0.0;
-    for (int e = threadIdx.x; e < (int)bs; e += MQE_NT) {
-        const T v00 = V00[e], v11 = V11[e];
-        acc += (double)c[e] * (0.5 * ((double)v11 - (double)v00)) +
-               0.5 * ((double)c[bs + e] * (double)V01[e] + (double)c[2 * bs + e] * (double)V10[e]);
-        if (Rout) Rout[e] = v00 + v11;
+    const int n = (int)bs;
+    // two elements of a thread per pass, loads before the store to Rout (as in
+    // pair_tensor; the clamped second index is skipped below)
+    for (int e0 = threadIdx.x; e0 < n; e0 += 2 * MQE_NT) {
+        T v00[2], v11[2], v01[2], v10[2], cz[2], cp[2], cm[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int e = min(e0 + u * MQE_NT, n - 1);
+            v00[u] = V00[e];
+            v11[u] = V11[e];
+            v01[u] = V01[e];
+            v10[u] = V10[e];
+            cz[u] = c[e];
+            cp[u] = c[bs + e];
+            cm[u] = c[2 * bs + e];
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int e = e0 + u * MQE_NT;
+            if (e < n) {
+                acc += (double)cz[u] * (0.5 * ((double)v11[u] - (double)v00[u])) +
+                       0.5 * ((double)cp[u] * (double)v01[u] + (double)cm[u] * (double)v10[u]);
+                if (Rout) Rout[e] = v00[u] + v11[u];
+            }
+        }
     }
     return block_sum(acc, C.red);
 }
